@@ -1,0 +1,127 @@
+/*
+ * zkgpu.h - C ABI of libzkgpu_hip.so, the MI355X (gfx950) BN254 prover core.
+ *
+ * Conventions
+ * -----------
+ * - Every entry returns 0 on success and a non-zero status on failure; the
+ *   message of the calling thread's last failure is zk_last_error().
+ * - Pointers named like device buffers are device addresses (from zk_alloc or
+ *   any HIP allocation); "host" pointers are ordinary host memory.
+ * - Field elements are 4 little-endian uint64 limbs (32 bytes). Rows called
+ *   "mont" hold a*2^256 mod m (Montgomery form, R = 2^256), the layout of the
+ *   reference's (n, 4) uint64 arrays (src/zkgpu/field256_gpu.py:12-20).
+ *   Rows called "plain" hold the canonical value itself.
+ * - Jacobian G1 points are 12 limbs (X, Y, Z), affine points 8 limbs (x, y),
+ *   coordinates in Montgomery form over the base field p; the identity is
+ *   (1, 1, 0) (src/zkgpu/curve_gpu.py:15-23).
+ * - "bcast" arguments are 1 when that operand is a single row broadcast
+ *   against the other (the reference's step-0 launch, field256_gpu.py:471).
+ * - stream is a hipStream_t (NULL = the null stream); work is enqueued
+ *   asynchronously unless the entry says it synchronises.
+ * - field ids: ZK_FIELD_FR (BN254 scalar field r) and ZK_FIELD_FQ (BN254
+ *   base field p) are built in with compile-time constants; other odd moduli
+ *   below 2^255 get an id from zk_field_register.
+ */
+#ifndef ZKGPU_H
+#define ZKGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZK_FIELD_FR 0
+#define ZK_FIELD_FQ 1
+
+/* ---- lifecycle, memory, streams (replaces src/zkgpu/gpu_backend.py:79-159:
+ *      CuPy detection, NVRTC compile and CuPy's allocator/streams) ---- */
+int zk_init(int device);
+const char* zk_last_error(void);
+int zk_device_count(int* count);
+int zk_device_info(char* name, int name_len, int* compute_units, int* clock_khz);
+int zk_mem_info(uint64_t* free_bytes, uint64_t* total_bytes);
+int zk_alloc(void** ptr, uint64_t bytes, void* stream);
+int zk_free(void* ptr);
+int zk_memcpy_h2d(void* dst, const void* host_src, uint64_t bytes, void* stream);  /* synchronous for host_src */
+int zk_memcpy_d2h(void* host_dst, const void* src, uint64_t bytes, void* stream);  /* synchronises stream */
+int zk_memcpy_d2d(void* dst, const void* src, uint64_t bytes, void* stream);
+int zk_memset(void* dst, int value, uint64_t bytes, void* stream);
+int zk_stream_create(void** stream);
+int zk_stream_destroy(void* stream);
+int zk_stream_sync(void* stream);
+int zk_device_sync(void);
+int zk_event_create(void** event);
+int zk_event_destroy(void* event);
+int zk_event_record(void* event, void* stream);
+int zk_event_elapsed_ms(void* start, void* end, float* ms);
+int zk_stream_wait_event(void* stream, void* event);
+/* Probe kernel: replaces the CuPy probe (gpu_backend.py:94-99). */
+int zk_selftest(void);
+
+/* ---- field arithmetic (replaces the kernels zk256_mul / zk256_add / zk256_sub,
+ *      src/zkgpu/field256_gpu.py:172-191, and Field256's Montgomery constants,
+ *      field256_gpu.py:277-288) ---- */
+/* Registers an odd modulus in [3, 2^255) (plain limbs) and returns its id. */
+int zk_field_register(const uint64_t modulus[4], int* field_id);
+/* mont[i] = (plain[i] mod m) * 2^256 mod m, for any plain[i] < 2^256 (Field256.from_ints, :297-310). */
+int zk_field_to_mont(int field_id, const uint64_t* plain, uint64_t* mont, int64_t n, void* stream);
+/* plain[i] = mont[i] * 2^-256 mod m (Field256.to_ints, :312-319). */
+int zk_field_from_mont(int field_id, const uint64_t* mont, uint64_t* plain, int64_t n, void* stream);
+/* out[i] = a[i] * b[i] * 2^-256, a[i] + b[i], a[i] - b[i] mod m (Field256.mul/add/sub, :325-341). */
+int zk_field_mul(int field_id, const uint64_t* a, int a_bcast, const uint64_t* b, int b_bcast, uint64_t* out, int64_t n, void* stream);
+int zk_field_add(int field_id, const uint64_t* a, int a_bcast, const uint64_t* b, int b_bcast, uint64_t* out, int64_t n, void* stream);
+int zk_field_sub(int field_id, const uint64_t* a, int a_bcast, const uint64_t* b, int b_bcast, uint64_t* out, int64_t n, void* stream);
+
+/* ---- NTT (replaces zk256_bit_reverse + zk256_ntt_stage and Field256.ntt / _ntt_tables,
+ *      src/zkgpu/field256_gpu.py:198-234,347-387,492-508; and the coset transforms of
+ *      Radix2Domain, src/zkgpu/domain.py:112-135) ---- */
+/* out = DFT_n(in) with omega (plain) a primitive 2^log_n-th root, natural order in and out.
+ * inverse != 0: in is transformed with omega and scaled by n^-1 (pass omega^-1 for an inverse NTT).
+ * coset (plain, or NULL): forward multiplies in[j] by coset^j first; inverse multiplies out[j]
+ * by coset^-j afterwards. in and out are mont rows; in may equal out. Tables are cached per
+ * (field, log_n, omega, coset). */
+int zk_ntt(int field_id, const uint64_t* in, uint64_t* out, int log_n, int inverse,
+           const uint64_t omega[4], const uint64_t* coset, void* stream);
+int zk_ntt_cache_clear(void);
+/* Groth16 quotient over Fr (demo/prover_simulation.py:142-150 with coset shift g):
+ * h = coset_intt((coset_ntt(intt a) * coset_ntt(intt b) - coset_ntt(intt c)) * (g^n - 1)^-1).
+ * a, b, c, h: n mont rows each (h may alias none of them). */
+int zk_groth16_quotient(const uint64_t* a, const uint64_t* b, const uint64_t* c, uint64_t* h,
+                        int log_n, const uint64_t shift[4], void* stream);
+/* Several Fr NTTs of different sizes in one batched launch sequence. Per job j: in[j], out[j]
+ * (device addresses), log_n[j], inverse[j]; the roots are the standard ones from generator 5. */
+int zk_ntt_batch(int count, const uint64_t* const* in, uint64_t* const* out, const int32_t* log_n,
+                 const int32_t* inverse, void* stream);
+
+/* ---- G1 point kernels (replaces zk_g1_add / zk_g1_add_affine / zk_g1_double,
+ *      src/zkgpu/curve_gpu.py:236-274; same EFD formulas, same (X, Y, Z)) ---- */
+int zk_g1_add(const uint64_t* a, int a_bcast, const uint64_t* b, int b_bcast, uint64_t* out, int64_t n, void* stream);
+int zk_g1_add_affine(const uint64_t* a, int a_bcast, const uint64_t* b_affine, int b_bcast, uint64_t* out, int64_t n, void* stream);
+int zk_g1_double(const uint64_t* a, uint64_t* out, int64_t n, void* stream);
+/* Replaces the host-side Point(x, y) validation in msm_pippenger_gpu (msm_gpu.py:441-442):
+ * *first_bad (host) = smallest i whose affine mont point is off y^2 = x^3 + 3, or -1. Synchronises. */
+int zk_g1_on_curve(const uint64_t* points_affine, int64_t n, int64_t* first_bad, void* stream);
+/* out_affine[i] = k[i] * G for plain scalars k[i] < 2^256 (all-zero rows for the identity);
+ * the SRS generator behind the device-resident basis (no counterpart: the reference builds
+ * its bases with Python scalar multiplications, benchmarks/bench_msm.py:40-44). */
+int zk_g1_fixed_base(const uint64_t* k, int64_t n, uint64_t* out_affine, void* stream);
+
+/* ---- Pippenger MSM (replaces msm_pippenger_gpu's device work and the CPU window
+ *      combination, src/zkgpu/msm_gpu.py:336-377,447-449 and msm.py:156-169) ---- */
+/* out_jac (device, 12 limbs) = sum_i s_i * P_i as a Jacobian mont point, for n affine mont points
+ * and plain scalars s_i < 2^256 (reduced mod r on the device). window: bits per signed digit
+ * (0 = automatic); chunk_len: bucket entries per thread in the accumulation (0 = automatic). */
+int zk_msm_g1(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window,
+              int chunk_len, uint64_t* out_jac, void* stream);
+/* count MSMs in one batched launch sequence; job j uses points_affine[j] (may all point into one
+ * shared basis), scalars[j], n[j] (host arrays of device addresses / sizes) and writes
+ * out_jac + 12 * j. window[j] may be 0 (automatic). workspace/workspace_bytes: reserved, pass NULL/0. */
+int zk_msm_g1_batch(int count, const uint64_t* const* points_affine, const uint64_t* const* scalars,
+                    const int64_t* n, const int32_t* window, uint64_t* out_jac, void* workspace,
+                    uint64_t* workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZKGPU_H */

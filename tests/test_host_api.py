@@ -1,0 +1,140 @@
+"""
+The drop-in API's host-side behaviour, checked without a GPU.
+
+Validation order and messages follow the reference (field256_gpu.py:250-291,
+347-368; curve_gpu.py:354-439; msm_gpu.py:385-446; curve.py:52-179). Without a
+GPU the compute entry points must raise RuntimeError: the package has no CPU
+fallback.
+"""
+
+import numpy as np
+import pytest
+
+from zkgpu import curve, field256_gpu, gpu_backend, msm_gpu
+from zkgpu.curve import JACOBIAN_IDENTITY, G, P, Point, from_jacobian, to_jacobian
+from zkgpu.field import FIELD_MODULUS as R
+from zkgpu.field256_gpu import BN254_FQ, BN254_FR, Field256
+from zkgpu.msm import _check_window_size, pippenger_window_size
+from zkgpu.ntt import get_ntt_root, quadratic_non_residue
+
+no_gpu = pytest.mark.skipif(gpu_backend.GPU_AVAILABLE, reason="checks the no-GPU behaviour")
+
+
+def test_constants_match_published_values():
+    assert BN254_FR._n_inv == 0xC2E1F593EFFFFFFF
+    assert BN254_FQ._n_inv == 0x87D20782E4866389
+    for f in (BN254_FR, BN254_FQ):
+        assert f._n_inv * f.modulus % 2**64 == 2**64 - 1
+        assert f._r2 == pow(2, 512, f.modulus)
+    assert repr(Field256(3)) == "Field256(0x3)"
+
+
+def test_source_carries_the_constants():
+    src = BN254_FR.cuda_source()
+    assert "#define ZK_NINV 0xc2e1f593efffffffULL" in src
+    assert "#define ZK_P0 0x43e1f593f0000001ULL" in src
+    assert "#define ZK_ONE0 0xac96341c4ffffffbULL" in src
+    assert src.index("#define") < src.index("mont_raw")
+
+
+@pytest.mark.parametrize("modulus", [2, 1, 0, -7, 2**64, 2**255 + 1, 2**256 - 189])
+def test_invalid_moduli(modulus):
+    with pytest.raises(ValueError, match="odd and in"):
+        Field256(modulus)
+
+
+def test_float_modulus():
+    with pytest.raises(TypeError):
+        Field256(7.0)
+
+
+@pytest.mark.parametrize(
+    "bad",
+    [np.zeros((3, 4), dtype=np.int64), np.zeros((3, 3), dtype=np.uint64), np.zeros(12, dtype=np.uint64), [[0, 0, 0, 0]]],
+    ids=["int64", "3-limbs", "flat", "list"],
+)
+def test_malformed_rows(bad):
+    with pytest.raises(TypeError, match=r"\(n, 4\) uint64"):
+        BN254_FR.mul(bad, np.zeros((1, 4), dtype=np.uint64))
+
+
+def test_broadcast_mismatch():
+    with pytest.raises(ValueError, match="broadcast"):
+        BN254_FR.mul(np.zeros((2, 4), dtype=np.uint64), np.zeros((3, 4), dtype=np.uint64))
+
+
+@pytest.mark.parametrize(
+    ("field", "n", "message"),
+    [(BN254_FR, 6, "power of 2"), (BN254_FQ, 4, "not divisible"), (Field256(2**255 - 1), 2, "odd prime")],
+)
+def test_ntt_validation_precedes_any_device_work(field, n, message):
+    with pytest.raises(ValueError, match=message):
+        field.ntt(np.zeros((n, 4), dtype=np.uint64))
+
+
+def test_roots():
+    assert quadratic_non_residue(R) == 5
+    assert get_ntt_root(2**28, R) == 19103219067921713944291392827692070036145651957329286315305642004821462161904
+    with pytest.raises(ValueError, match="power of 2"):
+        get_ntt_root(12, R)
+
+
+def test_window_model():
+    assert [pippenger_window_size(n) for n in (0, 100, 1_000, 10_000, 100_000)] == [1, 5, 7, 10, 13]
+    for bad in (0, 17):
+        with pytest.raises(ValueError, match="window_size"):
+            _check_window_size(bad)
+    with pytest.raises(TypeError):
+        _check_window_size(2.0)
+
+
+def test_msm_argument_validation():
+    with pytest.raises(ValueError, match="Length mismatch"):
+        msm_gpu.msm_pippenger_gpu([G.x], [G.y], [1, 2])
+    for bad in (0, 17):
+        with pytest.raises(ValueError, match="window_size"):
+            msm_gpu.msm_pippenger_gpu([G.x], [G.y], [1], window_size=bad)
+    with pytest.raises(TypeError, match="scalars must contain integers"):
+        msm_gpu.msm_pippenger_gpu([G.x], [G.y], [1.5])
+    with pytest.raises(TypeError):
+        msm_gpu.msm_pippenger_gpu([G.x], [G.y], [1], P)  # window_size is keyword-only
+    assert msm_gpu.msm_pippenger_gpu([], [], []) == Point.identity()
+
+
+def test_point_type():
+    assert Point(1, 2 + P) == G
+    assert -G == Point(1, P - 2)
+    with pytest.raises(ValueError, match="not on the BN254 curve"):
+        Point(1, 3)
+    with pytest.raises(ValueError, match="identity"):
+        Point(1, None)
+    with pytest.raises(TypeError):
+        Point(1.0, 2.0)
+    assert from_jacobian(JACOBIAN_IDENTITY).is_identity()
+    assert from_jacobian(to_jacobian(G)) == G
+    lam = 12345
+    assert from_jacobian((1 * lam**2 % P, 2 * lam**3 % P, lam)) == G
+    with pytest.raises(ValueError, match="no affine coordinates"):
+        Point.identity().affine()
+
+
+def test_affine_rows_reject_the_identity():
+    from zkgpu import curve_gpu
+
+    with pytest.raises(ValueError, match="point at infinity"):
+        curve_gpu.affine_to_device([G, Point.identity()])
+    with pytest.raises(ValueError, match="triples"):
+        curve_gpu.jacobian_to_device([(1, 2)])
+
+
+@no_gpu
+def test_no_cpu_fallback():
+    for call in (
+        lambda: BN254_FR.from_ints([1, 2]),
+        lambda: BN254_FR.mul(np.zeros((1, 4), dtype=np.uint64), np.zeros((1, 4), dtype=np.uint64)),
+        lambda: BN254_FR.ntt(np.zeros((4, 4), dtype=np.uint64)),
+        lambda: msm_gpu.msm_pippenger_gpu([G.x], [G.y], [3]),
+        lambda: G * 3,
+    ):
+        with pytest.raises(RuntimeError, match="no CPU fallback"):
+            call()

@@ -1,0 +1,323 @@
+// Prime-field arithmetic for gfx950 on nine 29-bit limbs.
+//
+// Representation. An element is nine 29-bit limbs (261 bits) in VGPRs. With
+// limbs below 2^29, each partial product is below 2^58, so a whole column of
+// a 9x9 product-scanning Montgomery multiplication (at most 18 products plus
+// the carry in) stays below 2^63 in one 64-bit accumulator: every partial
+// product is a single v_mad_u64_u32 with no carry handling. Montgomery
+// reduction is interleaved per column (finely integrated product scanning)
+// with R' = 2^261.
+//
+// Memory layout. Rows in HBM keep the reference's layout: four little-endian
+// 64-bit limbs of a * 2^256 mod p (the "256-form"). load_row/store_row repack
+// between eight 32-bit words and nine 29-bit limbs with funnel shifts.
+//
+// Forms. mont(x, y) = x * y * 2^-261 mod p. Values are tagged by the power of
+// two they carry: the 256-form a*2^256, the 261-form a*2^261 and plain a.
+// mont keeps a form when one operand is in the 261-form, so twiddles,
+// coset powers and curve coordinates inside kernels are kept in the 261-form,
+// and data that is only ever multiplied by such constants (NTT values) stays
+// in the 256-form with no conversion at all. The fixed multipliers c256,
+// c266, c517 and c522 (zk_constants.hpp) move values between forms.
+//
+// Reductions. mont returns a value below 2p for inputs below 2p (below
+// 2^257 for the BN254 moduli); canonical() subtracts p once. The arithmetic
+// below returns canonical values (in [0, p)) from canonical inputs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "zk_constants.hpp"
+
+namespace zk {
+
+constexpr int NL = 9;
+constexpr int LB = 29;
+constexpr uint32_t LMASK = (1u << LB) - 1;
+
+struct Fe {
+  uint32_t l[NL];
+};
+
+// A modulus chosen at run time (any odd p in [3, 2^255)), passed by value to
+// kernels; it has the same member names as FrConst / FqConst, so templates
+// read f.p[i] whether the field is built in or registered.
+struct FieldRT {
+  uint32_t p[NL];
+  uint32_t c256[NL];
+  uint32_t c266[NL];
+  uint32_t c517[NL];
+  uint32_t c522[NL];
+  uint32_t one[NL];
+  uint32_t p32[8];
+  uint32_t ninv;
+};
+
+template <class F>
+__device__ __forceinline__ Fe fe_const(const uint32_t (&c)[NL]) {
+  Fe r;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) r.l[i] = c[i];
+  return r;
+}
+
+__device__ __forceinline__ Fe fe_zero() {
+  Fe r;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) r.l[i] = 0;
+  return r;
+}
+
+__device__ __forceinline__ bool fe_is_zero(const Fe& a) {
+  uint32_t x = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) x |= a.l[i];
+  return x == 0;
+}
+
+__device__ __forceinline__ bool fe_eq(const Fe& a, const Fe& b) {
+  uint32_t x = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) x |= a.l[i] ^ b.l[i];
+  return x == 0;
+}
+
+// a - p if a >= p, for a < 2p.
+template <class F>
+__device__ __forceinline__ Fe canonical(const F& f, const Fe& a) {
+  Fe d;
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    uint32_t t = a.l[i] - f.p[i] - borrow;
+    borrow = t >> 31;  // limbs are below 2^29, so a wrap sets the top bit
+    d.l[i] = t & LMASK;
+  }
+  Fe r;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) r.l[i] = borrow ? a.l[i] : d.l[i];
+  return r;
+}
+
+// x * y * 2^-261 mod p, below 2p.
+template <class F>
+__device__ __forceinline__ Fe mont_raw(const F& f, const Fe& a, const Fe& b) {
+  uint32_t m[NL];
+  Fe r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+#pragma unroll
+    for (int i = 0; i <= k; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+    for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * f.p[k - i];
+    m[k] = ((uint32_t)acc * f.ninv) & LMASK;
+    acc += (uint64_t)m[k] * f.p[0];  // clears the low 29 bits
+    acc >>= LB;
+  }
+#pragma unroll
+  for (int k = NL; k < 2 * NL - 1; ++k) {
+#pragma unroll
+    for (int i = k - (NL - 1); i < NL; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+    for (int i = k - (NL - 1); i < NL; ++i) acc += (uint64_t)m[i] * f.p[k - i];
+    r.l[k - NL] = (uint32_t)acc & LMASK;
+    acc >>= LB;
+  }
+  r.l[NL - 1] = (uint32_t)acc;
+  return r;
+}
+
+// x^2 * 2^-261 mod p, below 2p: the 36 cross products are formed once against
+// a doubled operand (30-bit limbs, still far from overflowing a column).
+template <class F>
+__device__ __forceinline__ Fe mont_sqr_raw(const F& f, const Fe& a) {
+  uint32_t m[NL], d[NL];
+  Fe r;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) d[i] = a.l[i] << 1;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+#pragma unroll
+    for (int i = 0; 2 * i < k; ++i) acc += (uint64_t)a.l[i] * d[k - i];
+    if ((k & 1) == 0) acc += (uint64_t)a.l[k / 2] * a.l[k / 2];
+#pragma unroll
+    for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * f.p[k - i];
+    m[k] = ((uint32_t)acc * f.ninv) & LMASK;
+    acc += (uint64_t)m[k] * f.p[0];
+    acc >>= LB;
+  }
+#pragma unroll
+  for (int k = NL; k < 2 * NL - 1; ++k) {
+#pragma unroll
+    for (int i = k - (NL - 1); 2 * i < k; ++i) acc += (uint64_t)a.l[i] * d[k - i];
+    if ((k & 1) == 0) acc += (uint64_t)a.l[k / 2] * a.l[k / 2];
+#pragma unroll
+    for (int i = k - (NL - 1); i < NL; ++i) acc += (uint64_t)m[i] * f.p[k - i];
+    r.l[k - NL] = (uint32_t)acc & LMASK;
+    acc >>= LB;
+  }
+  r.l[NL - 1] = (uint32_t)acc;
+  return r;
+}
+
+template <class F>
+__device__ __forceinline__ Fe mont(const F& f, const Fe& a, const Fe& b) {
+  return canonical(f, mont_raw(f, a, b));
+}
+
+template <class F>
+__device__ __forceinline__ Fe mont_sqr(const F& f, const Fe& a) {
+  return canonical(f, mont_sqr_raw(f, a));
+}
+
+// (a + b) mod p for canonical a, b.
+template <class F>
+__device__ __forceinline__ Fe fadd(const F& f, const Fe& a, const Fe& b) {
+  Fe s;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    uint32_t t = a.l[i] + b.l[i] + c;
+    s.l[i] = t & LMASK;
+    c = t >> LB;
+  }
+  return canonical(f, s);  // s < 2p < 2^261, so the carry out of the top limb is 0
+}
+
+// (a - b) mod p for canonical a, b.
+template <class F>
+__device__ __forceinline__ Fe fsub(const F& f, const Fe& a, const Fe& b) {
+  Fe d;
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    uint32_t t = a.l[i] - b.l[i] - borrow;
+    borrow = t >> 31;
+    d.l[i] = t & LMASK;
+  }
+  // a < b: the difference wrapped modulo 2^261, so add p back (the carry out cancels the wrap)
+  uint32_t mask = 0u - borrow, c = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    uint32_t t = d.l[i] + (f.p[i] & mask) + c;
+    d.l[i] = t & LMASK;
+    c = t >> LB;
+  }
+  return d;
+}
+
+template <class F>
+__device__ __forceinline__ Fe fdbl(const F& f, const Fe& a) {
+  return fadd(f, a, a);
+}
+
+template <class F>
+__device__ __forceinline__ Fe fneg(const F& f, const Fe& a) {
+  return fsub(f, fe_zero(), a);
+}
+
+// ---- conversion between 8x32-bit words (memory) and 9x29-bit limbs ----
+
+__device__ __forceinline__ Fe from_words(const uint32_t w[8]) {
+  Fe r;
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int bit = LB * k, word = bit >> 5, sh = bit & 31;
+    uint32_t lo = w[word];
+    uint32_t hi = (word + 1 < 8) ? w[word + 1] : 0u;
+    uint32_t v = sh ? ((lo >> sh) | (hi << (32 - sh))) : lo;
+    r.l[k] = v & LMASK;
+  }
+  return r;
+}
+
+__device__ __forceinline__ void to_words(const Fe& a, uint32_t w[8]) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int bit = 32 * j;
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int lo = LB * k;  // limb k covers bits [lo, lo + 29)
+      if (lo + LB <= bit || lo >= bit + 32) continue;
+      if (lo >= bit) v |= a.l[k] << (lo - bit);
+      else v |= a.l[k] >> (bit - lo);
+    }
+    w[j] = v;
+  }
+}
+
+// Loads one 32-byte row (two 16-byte loads).
+__device__ __forceinline__ Fe load_row(const uint64_t* row) {
+  const uint4* p = reinterpret_cast<const uint4*>(row);
+  uint4 x = p[0], y = p[1];
+  uint32_t w[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+  return from_words(w);
+}
+
+__device__ __forceinline__ void store_row(uint64_t* row, const Fe& a) {
+  uint32_t w[8];
+  to_words(a, w);
+  uint4* p = reinterpret_cast<uint4*>(row);
+  p[0] = make_uint4(w[0], w[1], w[2], w[3]);
+  p[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// Canonical plain value < 2^256 from raw 32-bit words, reduced mod p, in the 256-form.
+template <class F>
+__device__ __forceinline__ Fe plain_to_256(const F& f, const Fe& x) {
+  return mont(f, x, fe_const<F>(f.c517));  // x * 2^517 * 2^-261 = x * 2^256, reduced
+}
+
+template <class F>
+__device__ __forceinline__ Fe form256_to_plain(const F& f, const Fe& x) {
+  Fe c = fe_zero();
+  c.l[0] = 32;  // x * 2^5 * 2^-261 = a * 2^256 * 2^-256
+  return mont(f, x, c);
+}
+
+template <class F>
+__device__ __forceinline__ Fe form256_to_261(const F& f, const Fe& x) {
+  return mont(f, x, fe_const<F>(f.c266));
+}
+
+template <class F>
+__device__ __forceinline__ Fe form261_to_256(const F& f, const Fe& x) {
+  return mont(f, x, fe_const<F>(f.c256));
+}
+
+// a * b in the 256-form for two 256-form operands: a * (b * 2^5) * 2^-261.
+template <class F>
+__device__ __forceinline__ Fe mul256(const F& f, const Fe& a, const Fe& b) {
+  return mont(f, a, mont(f, b, fe_const<F>(f.c266)));
+}
+
+// x^e for a 261-form x and a 256-bit exponent (plain words, little-endian), in the 261-form.
+template <class F>
+__device__ __forceinline__ Fe fpow(const F& f, const Fe& x, const uint32_t e[8]) {
+  Fe acc = fe_const<F>(f.one);
+  for (int i = 255; i >= 0; --i) {
+    acc = mont_sqr(f, acc);
+    if ((e[i >> 5] >> (i & 31)) & 1u) acc = mont(f, acc, x);
+  }
+  return acc;
+}
+
+// x^-1 for a 261-form x != 0 (Fermat), in the 261-form.
+template <class F>
+__device__ __forceinline__ Fe finv(const F& f, const Fe& x) {
+  uint32_t e[8];
+  uint32_t borrow = 2;  // e = p - 2
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    uint32_t w = f.p32[i];
+    e[i] = w - borrow;
+    borrow = w < borrow ? 1u : 0u;
+  }
+  return fpow(f, x, e);
+}
+
+}  // namespace zk

@@ -1,0 +1,404 @@
+// Radix-2 NTT over LDS-staged multi-stage passes (include/zkgpu.h, "NTT").
+//
+// Algorithm. Decimation in frequency (Gentleman-Sande): stage s pairs rows
+// i and i + n/2^(s+1) and maps (u, v) to (u + v, (u - v) * w) with
+// w = omega^(j * 2^s), j = i mod n/2^(s+1). After log2(n) stages row i holds
+// the transform at bitrev(i); the last pass writes there directly, so input
+// and output are both in natural order, as in the reference's transform
+// (src/zkgpu/ntt.py:149-203 through field256_gpu.py:375-387). Every value is
+// fully reduced, so the output equals the DFT bit for bit whatever the
+// butterfly order.
+//
+// Passes. A pass runs k consecutive stages on independent groups of 2^k
+// rows (stride S = n / 2^(s0 + k)); a workgroup stages a tile of C groups x
+// 2^k rows (1024 rows) in LDS as nine 29-bit limb planes, so consecutive
+// threads touch consecutive banks. Groups of a tile are C consecutive columns,
+// so every global row access of a wavefront covers C consecutive rows. The
+// last pass takes groups that are far apart in index, so that after bit
+// reversal its writes are again C consecutive rows. 2^22 runs as 8 + 7 + 7
+// stages: three read+write sweeps of HBM instead of the reference's 23.
+//
+// Twiddles. One table per stage, w_s[j] = omega^(j * 2^s), so that a pass
+// reads each stage's twiddles contiguously; values are in the 261-form, so
+// (u - v) * w keeps the data's 256-form without conversions. Coset powers and
+// n^-1 are tables in the same form, applied as the first pass loads and the
+// last pass stores. Tables are built on the device from two short seed tables
+// (host powers of omega) and cached per (field, log n, omega, coset).
+#include <list>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/zkgpu.h"
+#include "zk_fields.hpp"
+#include "zk_host_math.hpp"
+#include "zk_ntt.hpp"
+
+namespace zk {
+
+constexpr int kTile = 1024;  // rows per workgroup tile
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ void lds_put(uint32_t* lds, int slot, const Fe& x) {
+#pragma unroll
+  for (int i = 0; i < NL; ++i) lds[i * kTile + slot] = x.l[i];
+}
+
+__device__ __forceinline__ Fe lds_get(const uint32_t* lds, int slot) {
+  Fe x;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) x.l[i] = lds[i * kTile + slot];
+  return x;
+}
+
+__device__ __forceinline__ int64_t bitrev(int64_t i, int bits) {
+  return (int64_t)(__builtin_bitreverse64((uint64_t)i) >> (64 - bits));
+}
+
+// Row index of (group, m) in this pass: group -> (hi, lo), i = hi*2^(L-s0) + m*S + lo.
+struct PassGeom {
+  int log_n, s0, k, log_c;
+  int last;
+  __device__ __forceinline__ int64_t row(int64_t wg, int g, int m) const {
+    const int log_s = log_n - s0 - k;
+    int64_t hi, lo;
+    if (last) {  // S == 1: groups w + g * 2^(L-k-c) so that bit-reversed writes are contiguous
+      hi = wg + ((int64_t)g << (log_n - k - log_c));
+      lo = 0;
+    } else {
+      int64_t gid = (wg << log_c) + g;
+      hi = gid >> log_s;
+      lo = gid & ((1LL << log_s) - 1);
+    }
+    return (hi << (log_n - s0)) + ((int64_t)m << log_s) + lo;
+  }
+};
+
+template <class F>
+__global__ void __launch_bounds__(kThreads) k_ntt_pass(F f, const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
+                                                       const uint64_t* __restrict__ tw, PassGeom geo,
+                                                       const uint64_t* __restrict__ pre, const uint64_t* __restrict__ post,
+                                                       int post_is_table, QuotientArgs q) {
+  __shared__ uint32_t lds[NL * kTile];
+  const int C = 1 << geo.log_c;
+  const int T = C << geo.k;
+  const int64_t n = 1LL << geo.log_n;
+  const int64_t wg = blockIdx.x;
+  const int log_s = geo.log_n - geo.s0 - geo.k;
+
+  for (int e = threadIdx.x; e < T; e += kThreads) {
+    const int g = e & (C - 1), m = e >> geo.log_c;
+    const int64_t i = geo.row(wg, g, m);
+    Fe x;
+    if (q.a) {
+      // Groth16 numerator on the coset: (A*B' - C) * z^-1, with B' carrying an extra 2^5
+      // so that mont(A, B') is in the 256-form.
+      Fe av = load_row(q.a + 4 * i), bv = load_row(q.b + 4 * i), cv = load_row(q.c + 4 * i);
+      Fe zi = load_row(q.z_inv);
+      x = mont(f, fsub(f, mont(f, av, bv), cv), zi);
+    } else {
+      x = load_row(src + 4 * i);
+    }
+    if (pre) x = mont(f, x, load_row(pre + 4 * i));
+    lds_put(lds, e, x);
+  }
+  __syncthreads();
+
+  for (int t = 0; t < geo.k; ++t) {
+    const int hb = geo.k - 1 - t;  // bit of m that this stage pairs
+    const int s = geo.s0 + t;
+    const uint64_t* tws = tw + 4 * (n - (n >> s));  // stage s table: n >> (s + 1) entries
+    for (int b = threadIdx.x; b < (T >> 1); b += kThreads) {
+      const int g = b & (C - 1), qv = b >> geo.log_c;
+      const int mlow = qv & ((1 << hb) - 1);
+      const int m0 = ((qv >> hb) << (hb + 1)) | mlow;
+      const int m1 = m0 | (1 << hb);
+      const int64_t row0 = geo.row(wg, g, m0);
+      const int64_t j = ((int64_t)mlow << log_s) + (row0 & ((1LL << log_s) - 1));
+      Fe w = load_row(tws + 4 * j);
+      const int s0slot = (m0 << geo.log_c) + g, s1slot = (m1 << geo.log_c) + g;
+      Fe u = lds_get(lds, s0slot), v = lds_get(lds, s1slot);
+      lds_put(lds, s0slot, fadd(f, u, v));
+      lds_put(lds, s1slot, mont(f, fsub(f, u, v), w));
+    }
+    __syncthreads();
+  }
+
+  for (int e = threadIdx.x; e < T; e += kThreads) {
+    const int g = e & (C - 1), m = e >> geo.log_c;
+    const int64_t i = geo.row(wg, g, m);
+    Fe x = lds_get(lds, e);
+    const int64_t o = geo.last ? bitrev(i, geo.log_n) : i;
+    if (post) x = mont(f, x, load_row(post + (post_is_table ? 4 * o : 0)));
+    store_row(dst + 4 * o, x);
+  }
+}
+
+// Table expansion: out[j] = seed_lo[e & mask] * seed_hi[e >> h] for e = j << shift, in the 261-form.
+// seed tables hold plain values; c522 lifts them to the 261-form.
+template <class F>
+__global__ void __launch_bounds__(256) k_expand(F f, const uint64_t* __restrict__ lo_tab, const uint64_t* __restrict__ hi_tab,
+                                                int h, int shift, uint64_t* __restrict__ out, int64_t count,
+                                                const uint64_t* __restrict__ scale) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int64_t e = j << shift;
+  const Fe lift = fe_const<F>(f.c522);
+  Fe a = mont(f, load_row(lo_tab + 4 * (e & ((1LL << h) - 1))), lift);
+  Fe b = mont(f, load_row(hi_tab + 4 * (e >> h)), lift);
+  Fe x = mont(f, a, b);
+  if (scale) x = mont(f, x, mont(f, load_row(scale), lift));
+  store_row(out + 4 * j, x);
+}
+
+// ---- host side -----------------------------------------------------------
+
+namespace {
+
+struct TableKey {
+  int field, log_n, kind;  // 0: stage twiddles; 1: coset powers; 2: inverse post-scale table;
+                           // 3: one row extra * n^-1; 4: one row holding the constant a
+  uint64_t a[4], b[4];
+  bool operator<(const TableKey& o) const {
+    return std::tie(field, log_n, kind, a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]) <
+           std::tie(o.field, o.log_n, o.kind, o.a[0], o.a[1], o.a[2], o.a[3], o.b[0], o.b[1], o.b[2], o.b[3]);
+  }
+  bool operator==(const TableKey& o) const { return !(*this < o) && !(o < *this); }
+};
+
+struct Table {
+  uint64_t* ptr;
+  size_t rows;
+};
+
+std::mutex g_tab_mu;
+std::map<TableKey, Table> g_tables;
+std::list<TableKey> g_lru;
+constexpr size_t kMaxTables = 48;
+
+bool field_modulus(int field_id, U256* p) {
+  static const uint64_t r[4] = {0x43e1f593f0000001ULL, 0x2833e84879b97091ULL, 0xb85045b68181585dULL, 0x30644e72e131a029ULL};
+  static const uint64_t q[4] = {0x3c208c16d87cfd47ULL, 0x97816a916871ca8dULL, 0xb85045b68181585dULL, 0x30644e72e131a029ULL};
+  if (field_id == 0) { *p = U256::from(r); return true; }
+  if (field_id == 1) { *p = U256::from(q); return true; }
+  FieldRT f;
+  if (!lookup_field(field_id, &f)) return false;
+  for (int i = 0; i < 4; ++i) p->w[i] = (uint64_t)f.p32[2 * i] | ((uint64_t)f.p32[2 * i + 1] << 32);
+  return true;
+}
+
+// Expands count entries of base^(j << shift) (times scale, if given) into out, on the device.
+int expand(int field_id, const HostField& hf, const U256& base_plain, int log_count, int shift, uint64_t* out, int64_t count,
+           const U256* scale_plain, hipStream_t stream) {
+  const int bits = log_count + shift;  // exponents are below 2^bits
+  const int h = (bits + 1) / 2;
+  const int64_t nlo = 1LL << h, nhi = (1LL << (bits - h)) + 1;
+  std::vector<uint64_t> host((size_t)(nlo + nhi + 1) * 4);
+  U256 bm = hf.to_mont(base_plain), acc = hf.one();
+  for (int64_t i = 0; i < nlo; ++i) {
+    U256 v = hf.from_mont(acc);
+    memcpy(&host[(size_t)i * 4], v.w, 32);
+    acc = hf.mul(acc, bm);
+  }
+  U256 step = acc;  // base^(2^h)
+  acc = hf.one();
+  for (int64_t i = 0; i < nhi; ++i) {
+    U256 v = hf.from_mont(acc);
+    memcpy(&host[(size_t)(nlo + i) * 4], v.w, 32);
+    acc = hf.mul(acc, step);
+  }
+  if (scale_plain) memcpy(&host[(size_t)(nlo + nhi) * 4], scale_plain->w, 32);
+  Scratch sc(stream);
+  uint64_t* dev = sc.get<uint64_t>(host.size());
+  if (!dev) return 1;
+  ZK_HIP(hipMemcpyAsync(dev, host.data(), host.size() * 8, hipMemcpyHostToDevice, stream));
+  int rc = with_field(field_id, [&](auto f) {
+    hipLaunchKernelGGL((k_expand<decltype(f)>), dim3(grid_for(count, 256)), dim3(256), 0, stream, f, dev, dev + 4 * nlo, h,
+                       shift, out, count, scale_plain ? dev + 4 * (nlo + nhi) : nullptr);
+    ZK_LAUNCH_CHECK("table expansion");
+    return 0;
+  });
+  if (rc) return rc;
+  ZK_HIP(hipStreamSynchronize(stream));  // the host seed buffer is released on return
+  return 0;
+}
+
+// Returns a cached table, building it on first use.
+int get_table(const TableKey& key, hipStream_t stream, const uint64_t** out) {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  auto it = g_tables.find(key);
+  if (it != g_tables.end()) {
+    g_lru.remove(key);
+    g_lru.push_front(key);
+    *out = it->second.ptr;
+    return 0;
+  }
+  U256 p;
+  if (!field_modulus(key.field, &p)) return fail("unknown field id");
+  HostField hf(p);
+  const int64_t n = 1LL << key.log_n;
+  size_t rows = key.kind == 0 ? (size_t)(n > 1 ? n - 1 : 1) : key.kind >= 3 ? 1 : (size_t)n;
+  uint64_t* buf = nullptr;
+  if (g_tables.size() >= kMaxTables) {
+    // Evict the least recently used table once no queued work can still read it.
+    ZK_HIP(hipDeviceSynchronize());
+    TableKey victim = g_lru.back();
+    g_lru.pop_back();
+    (void)hipFree(g_tables[victim].ptr);
+    g_tables.erase(victim);
+  }
+  ZK_HIP(hipMalloc(&buf, rows * 32));
+  int rc = 0;
+  if (key.kind == 0) {
+    U256 omega = U256::from(key.a);
+    for (int s = 0; s < key.log_n && rc == 0; ++s) {
+      const int64_t off = n - (n >> s), cnt = n >> (s + 1);
+      rc = expand(key.field, hf, omega, key.log_n - 1 - s, s, buf + 4 * off, cnt, nullptr, stream);
+    }
+  } else if (key.kind == 1) {
+    rc = expand(key.field, hf, U256::from(key.b), key.log_n, 0, buf, n, nullptr, stream);
+  } else if (key.kind == 4) {
+    U256 value = hf.reduce(U256::from(key.a));
+    rc = expand(key.field, hf, U256::of(1), 0, 0, buf, 1, &value, stream);
+  } else if (key.kind == 3) {
+    U256 scale = hf.mulp(hf.invp(U256::of((uint64_t)n)), U256::from(key.a));
+    rc = expand(key.field, hf, U256::of(1), 0, 0, buf, 1, &scale, stream);
+  } else {
+    // extra * n^-1 * g^-j
+    U256 g = U256::from(key.b);
+    U256 ginv = hf.invp(g);
+    U256 ninv = hf.invp(U256::of((uint64_t)n));
+    U256 scale = hf.mulp(ninv, U256::from(key.a));  // a carries an extra factor (1 or 2^5)
+    rc = expand(key.field, hf, ginv, key.log_n, 0, buf, n, &scale, stream);
+  }
+  if (rc) {
+    (void)hipFree(buf);
+    return rc;
+  }
+  g_tables[key] = Table{buf, rows};
+  g_lru.push_front(key);
+  *out = buf;
+  return 0;
+}
+
+struct PassPlan {
+  int s0, k, log_c;
+};
+
+std::vector<PassPlan> plan_passes(int log_n) {
+  std::vector<PassPlan> out;
+  if (log_n <= 10) {
+    out.push_back({0, log_n, 0});
+    return out;
+  }
+  const int max_k = 8;
+  int passes = (log_n + max_k - 1) / max_k;
+  int s0 = 0;
+  for (int p = 0; p < passes; ++p) {
+    int k = (log_n - s0) / (passes - p);
+    if ((log_n - s0) % (passes - p)) ++k;
+    int log_c = 10 - k;
+    if (log_c > log_n - k) log_c = log_n - k;  // no more groups than exist
+    out.push_back({s0, k, log_c});
+    s0 += k;
+  }
+  return out;
+}
+
+}  // namespace
+
+int ntt_cache_clear() {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  ZK_HIP(hipDeviceSynchronize());
+  for (auto& kv : g_tables) (void)hipFree(kv.second.ptr);
+  g_tables.clear();
+  g_lru.clear();
+  return 0;
+}
+
+int twiddles(int field_id, int log_n, const uint64_t omega[4], hipStream_t stream, const uint64_t** out) {
+  TableKey key{field_id, log_n, 0, {omega[0], omega[1], omega[2], omega[3]}, {0, 0, 0, 0}};
+  return get_table(key, stream, out);
+}
+
+int coset_table(int field_id, int log_n, const uint64_t g[4], bool inverse, const uint64_t extra[4], hipStream_t stream,
+                const uint64_t** out) {
+  TableKey key{field_id, log_n, inverse ? 2 : 1, {extra[0], extra[1], extra[2], extra[3]}, {g[0], g[1], g[2], g[3]}};
+  return get_table(key, stream, out);
+}
+
+int scale_row(int field_id, int log_n, const uint64_t extra[4], hipStream_t stream, const uint64_t** out) {
+  TableKey key{field_id, log_n, 3, {extra[0], extra[1], extra[2], extra[3]}, {0, 0, 0, 0}};
+  return get_table(key, stream, out);
+}
+
+int const_row(int field_id, const uint64_t value[4], hipStream_t stream, const uint64_t** out) {
+  TableKey key{field_id, 0, 4, {value[0], value[1], value[2], value[3]}, {0, 0, 0, 0}};
+  return get_table(key, stream, out);
+}
+
+int ntt_run(int field_id, const uint64_t* in, uint64_t* out, int log_n, const uint64_t* tw, const uint64_t* pre,
+            const uint64_t* post, bool post_table, const QuotientArgs& q, hipStream_t stream) {
+  const int64_t n = 1LL << log_n;
+  std::vector<PassPlan> plan = plan_passes(log_n);
+  Scratch sc(stream);
+  uint64_t* tmp = nullptr;
+  if (plan.size() > 1) {
+    tmp = sc.get<uint64_t>((size_t)n * 4);
+    if (!tmp) return 1;
+  }
+  return with_field(field_id, [&](auto f) {
+    for (size_t p = 0; p < plan.size(); ++p) {
+      const bool first = p == 0, last = p + 1 == plan.size();
+      PassGeom geo{log_n, plan[p].s0, plan[p].k, plan[p].log_c, last ? 1 : 0};
+      const uint64_t* src = first ? in : tmp;
+      uint64_t* dst = last ? out : tmp;
+      const int64_t tiles = n >> (plan[p].k + plan[p].log_c);
+      QuotientArgs qa = first ? q : QuotientArgs{};
+      hipLaunchKernelGGL((k_ntt_pass<decltype(f)>), dim3((unsigned)tiles), dim3(kThreads), 0, stream, f, src, dst, tw, geo,
+                         first ? pre : nullptr, last ? post : nullptr, post_table ? 1 : 0, qa);
+      ZK_LAUNCH_CHECK("ntt pass");
+    }
+    return 0;
+  });
+}
+
+}  // namespace zk
+
+using namespace zk;
+
+extern "C" {
+
+int zk_ntt_cache_clear(void) { return ntt_cache_clear(); }
+
+int zk_ntt(int field_id, const uint64_t* in, uint64_t* out, int log_n, int inverse, const uint64_t omega[4],
+           const uint64_t* coset, void* stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  if (log_n < 0 || log_n > 28) return fail("log_n must be in [0, 28]");
+  const int64_t n = 1LL << log_n;
+  if (log_n == 0) {
+    // A single value is its own transform; the inverse scales by 1^-1 = 1 and g^0 = 1.
+    if (in != out) ZK_HIP(hipMemcpyAsync(out, in, 32, hipMemcpyDeviceToDevice, stream));
+    return 0;
+  }
+  const uint64_t* tw = nullptr;
+  if (int rc = twiddles(field_id, log_n, omega, stream, &tw)) return rc;
+  const uint64_t* pre = nullptr;
+  const uint64_t* post = nullptr;
+  const uint64_t one[4] = {1, 0, 0, 0};
+  if (!inverse && coset) {
+    if (int rc = coset_table(field_id, log_n, coset, false, one, stream, &pre)) return rc;
+  }
+  bool post_table = false;
+  if (inverse && coset) {
+    if (int rc = coset_table(field_id, log_n, coset, true, one, stream, &post)) return rc;
+    post_table = true;
+  } else if (inverse) {
+    if (int rc = scale_row(field_id, log_n, one, stream, &post)) return rc;
+  }
+  (void)n;
+  return ntt_run(field_id, in, out, log_n, tw, pre, post, post_table, QuotientArgs{}, stream);
+}
+
+}  // extern "C"

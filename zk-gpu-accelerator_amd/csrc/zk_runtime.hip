@@ -1,0 +1,296 @@
+// Lifecycle, memory, streams and events of the C ABI (include/zkgpu.h).
+#include "zk_runtime.hpp"
+
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/zkgpu.h"
+
+namespace zk {
+
+static thread_local std::string g_last_error;
+
+int fail(const std::string& msg) {
+  g_last_error = msg;
+  return 1;
+}
+
+int fail_hip(hipError_t err, const char* what) {
+  return fail(std::string(what) + ": " + hipGetErrorString(err));
+}
+
+// ---- caching allocator -------------------------------------------------
+// Size classes: powers of two up to 1 MiB, then multiples of 2 MiB.
+static size_t size_class(size_t bytes) {
+  if (bytes <= 256) return 256;
+  if (bytes <= (1u << 20)) {
+    size_t c = 256;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  const size_t g = size_t(2) << 20;
+  return (bytes + g - 1) / g * g;
+}
+
+struct Block {
+  size_t bytes;
+  hipStream_t stream;
+};
+
+static std::mutex g_alloc_mu;
+static std::unordered_map<void*, Block> g_live;
+static std::map<std::pair<hipStream_t, size_t>, std::vector<void*>> g_free;
+static size_t g_cached_bytes = 0;
+
+static void release_cache_locked() {
+  for (auto& kv : g_free)
+    for (void* p : kv.second) (void)hipFree(p);
+  g_free.clear();
+  g_cached_bytes = 0;
+}
+
+void* dev_alloc(size_t bytes, hipStream_t stream) {
+  size_t c = size_class(bytes);
+  {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    auto it = g_free.find({stream, c});
+    if (it != g_free.end() && !it->second.empty()) {
+      void* p = it->second.back();
+      it->second.pop_back();
+      g_cached_bytes -= c;
+      g_live[p] = Block{c, stream};
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, c);
+  if (e != hipSuccess) {
+    // Out of memory with blocks parked in the cache: give them back and retry once.
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    (void)hipGetLastError();
+    (void)hipDeviceSynchronize();
+    release_cache_locked();
+    e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+      fail_hip(e, "hipMalloc");
+      return nullptr;
+    }
+  }
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  g_live[p] = Block{c, stream};
+  return p;
+}
+
+void dev_free(void* ptr) {
+  if (!ptr) return;
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  auto it = g_live.find(ptr);
+  if (it == g_live.end()) return;
+  Block b = it->second;
+  g_live.erase(it);
+  g_free[{b.stream, b.bytes}].push_back(ptr);
+  g_cached_bytes += b.bytes;
+}
+
+Scratch::~Scratch() {
+  for (int i = 0; i < count_; ++i) dev_free(ptrs_[i]);
+}
+
+void* Scratch::raw(size_t bytes) {
+  if (count_ == kMax) {
+    fail("internal: too many scratch buffers in one call");
+    return nullptr;
+  }
+  void* p = dev_alloc(bytes ? bytes : 1, stream_);
+  if (p) ptrs_[count_++] = p;
+  return p;
+}
+
+static int g_cus = 0;
+int device_cus() { return g_cus > 0 ? g_cus : 256; }
+
+__global__ void probe_kernel(int64_t* x, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x[i] + 1;
+}
+
+}  // namespace zk
+
+using namespace zk;
+
+extern "C" {
+
+const char* zk_last_error(void) { return g_last_error.c_str(); }
+
+int zk_device_count(int* count) {
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    (void)hipGetLastError();
+    return fail_hip(e, "hipGetDeviceCount");
+  }
+  return 0;
+}
+
+int zk_init(int device) {
+  ZK_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  ZK_HIP(hipGetDeviceProperties(&prop, device));
+  g_cus = prop.multiProcessorCount;
+  return 0;
+}
+
+int zk_device_info(char* name, int name_len, int* compute_units, int* clock_khz) {
+  int dev = 0;
+  ZK_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  ZK_HIP(hipGetDeviceProperties(&prop, dev));
+  if (name && name_len > 0) {
+    snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (compute_units) *compute_units = prop.multiProcessorCount;
+  if (clock_khz) {
+    int clk = 0;
+    ZK_HIP(hipDeviceGetAttribute(&clk, hipDeviceAttributeClockRate, dev));
+    *clock_khz = clk;
+  }
+  return 0;
+}
+
+int zk_mem_info(uint64_t* free_bytes, uint64_t* total_bytes) {
+  size_t f = 0, t = 0;
+  ZK_HIP(hipMemGetInfo(&f, &t));
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  *free_bytes = (uint64_t)f + g_cached_bytes;  // cached blocks are free to this library
+  *total_bytes = (uint64_t)t;
+  return 0;
+}
+
+int zk_alloc(void** ptr, uint64_t bytes, void* stream) {
+  *ptr = dev_alloc((size_t)bytes, (hipStream_t)stream);
+  return *ptr ? 0 : 1;
+}
+
+int zk_free(void* ptr) {
+  dev_free(ptr);
+  return 0;
+}
+
+int zk_memcpy_h2d(void* dst, const void* src, uint64_t bytes, void* stream) {
+  if (!bytes) return 0;
+  ZK_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  ZK_HIP(hipStreamSynchronize((hipStream_t)stream));  // the host buffer may be released after return
+  return 0;
+}
+
+int zk_memcpy_d2h(void* dst, const void* src, uint64_t bytes, void* stream) {
+  if (!bytes) return 0;
+  ZK_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  ZK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+
+int zk_memcpy_d2d(void* dst, const void* src, uint64_t bytes, void* stream) {
+  if (!bytes) return 0;
+  ZK_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return 0;
+}
+
+int zk_memset(void* dst, int value, uint64_t bytes, void* stream) {
+  if (!bytes) return 0;
+  ZK_HIP(hipMemsetAsync(dst, value, (size_t)bytes, (hipStream_t)stream));
+  return 0;
+}
+
+int zk_stream_create(void** stream) {
+  hipStream_t s;
+  ZK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void*)s;
+  return 0;
+}
+
+int zk_stream_destroy(void* stream) {
+  if (!stream) return 0;
+  {
+    // Blocks cached for this stream can no longer be ordered by it.
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    for (auto it = g_free.begin(); it != g_free.end();) {
+      if (it->first.first == (hipStream_t)stream) {
+        for (void* p : it->second) {
+          g_free[{nullptr, it->first.second}].push_back(p);
+        }
+        it = g_free.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  ZK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  ZK_HIP(hipStreamDestroy((hipStream_t)stream));
+  return 0;
+}
+
+int zk_stream_sync(void* stream) {
+  ZK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+
+int zk_device_sync(void) {
+  ZK_HIP(hipDeviceSynchronize());
+  return 0;
+}
+
+int zk_event_create(void** event) {
+  hipEvent_t e;
+  ZK_HIP(hipEventCreate(&e));
+  *event = (void*)e;
+  return 0;
+}
+
+int zk_event_destroy(void* event) {
+  if (event) ZK_HIP(hipEventDestroy((hipEvent_t)event));
+  return 0;
+}
+
+int zk_event_record(void* event, void* stream) {
+  ZK_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+  return 0;
+}
+
+int zk_event_elapsed_ms(void* start, void* end, float* ms) {
+  ZK_HIP(hipEventSynchronize((hipEvent_t)end));
+  ZK_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+  return 0;
+}
+
+int zk_stream_wait_event(void* stream, void* event) {
+  ZK_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+  return 0;
+}
+
+int zk_selftest(void) {
+  const int n = 4;
+  int64_t host[n] = {0, 1, 2, 3};
+  int64_t* dev = nullptr;
+  ZK_HIP(hipMalloc(&dev, sizeof(host)));
+  hipError_t e = hipMemcpy(dev, host, sizeof(host), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(probe_kernel, dim3(1), dim3(64), 0, 0, dev, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(host, dev, sizeof(host), hipMemcpyDeviceToHost);
+  (void)hipFree(dev);
+  if (e != hipSuccess) return fail_hip(e, "probe kernel");
+  for (int i = 0; i < n; ++i)
+    if (host[i] != i + 1) return fail("probe kernel ran but returned wrong values");
+  return 0;
+}
+
+}  // extern "C"
