@@ -1,0 +1,289 @@
+#!/usr/bin/env python3
+"""
+bench.py: the BN254 prover core on one MI355X.
+
+A step is one plain G1 MSM over 2^20 device-resident points P_i = k_i G
+(seeded k_i in [1, r)) with witness-like scalars (50% uniform in [0, r), 30% in
+{0, 1}, 20% below 2^32), through zk_msm_g1: the headline G1 MSM Mpts/s at 2^20.
+The same run measures the Fr NTT at 2^22 (Melem/s, forward), the Groth16
+quotient pipeline at 2^22, the batched form against the plain per-call path
+on the same mixed-size workload, and the served form's p50/p99 latency, and
+prints ONE JSON line.
+
+roofline: the dominant kernel of a step is the bucket accumulation
+(msm_accumulate); its duration comes from HIP events on its own stream around
+each launch (zk_prof_*). Algorithmic work per launch = (nonzero signed
+digits) x 11 Fq multiplications x 136 32-bit multiply-adds, the cost basis of
+a mixed addition on 8x32-bit limbs; the peak is gfx950's measured
+v_mad_u64_u32 rate (zk_valu_peak). cpu_baseline: the oracle's Python-int
+msm_pippenger (the reference's execution model: one core) on a 2^12-point
+sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "zk-gpu-accelerator_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def signed_digit_count(scalars: np.ndarray, c: int) -> int:
+    """Nonzero signed c-bit digits over all scalars (the accumulation's unit count)."""
+    windows = 254 // c + 1
+    s = np.ascontiguousarray(scalars, dtype=np.uint64)
+    carry = np.zeros(s.shape[0], dtype=np.int64)
+    half = 1 << (c - 1)
+    total = 0
+    for w in range(windows):
+        bit = w * c
+        word, sh = bit // 64, bit % 64
+        raw = (s[:, word] >> np.uint64(sh)) if word < 4 else np.zeros(s.shape[0], dtype=np.uint64)
+        if sh + c > 64 and word + 1 < 4:
+            raw = raw | (s[:, word + 1] << np.uint64(64 - sh))
+        raw = (raw & np.uint64((1 << c) - 1)).astype(np.int64) + carry
+        carry = (raw > half).astype(np.int64)
+        digit = raw - carry * (1 << c)
+        total += int(np.count_nonzero(digit))
+    return total
+
+
+def timed(fn, steps: int, warmup: int, gb) -> float:
+    """Milliseconds per step over exactly `steps` calls, synchronised on both sides."""
+    for _ in range(warmup):
+        fn()
+    gb.synchronize()
+    start, end = gb.Event(), gb.Event()
+    start.record()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    end.record()
+    gb.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3
+    return max(start.elapsed_ms(end), 1e-9) / steps if steps else wall
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log-msm", type=int, default=20)
+    ap.add_argument("--log-ntt", type=int, default=22)
+    ap.add_argument("--no-extras", action="store_true", help="only the headline MSM")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+            "run with --gpus 1",
+            file=sys.stderr,
+        )
+        return 2
+
+    from zkgpu import gpu_backend as gb
+    from zkgpu import workloads
+    from zkgpu.field256_gpu import BN254_FR
+    from zkgpu.msm_gpu import msm_g1_jacobian
+
+    gb.require_gpu("bench")
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+
+    # ---- headline: plain MSM at 2^20 --------------------------------------
+    n = 1 << args.log_msm
+    log(f"setup: basis of {n} points k_i G on the GPU, witness-like scalars")
+    keys = workloads.key_rows(n, 0x5EED04)
+    points = workloads.srs_points(keys)
+    scalars_host = workloads.witness_rows(n, 0x5EED04)
+    scalars = gb.asarray(scalars_host)
+    out = gb.empty((3, 4))
+    window = int(os.environ.get("ZK_MSM_WINDOW", "0"))
+
+    def msm_step() -> None:
+        msm_g1_jacobian(points, scalars, out, window_size=window)
+
+    log("msm: warmup + timed")
+    gb.prof_enable(True)
+    for _ in range(args.warmup):
+        msm_step()
+    gb.synchronize()
+    gb.prof_read("msm_accumulate", reset=True)
+    ms = timed(msm_step, args.steps, 0, gb)
+    acc_ms, acc_launches = gb.prof_read("msm_accumulate")
+    fin_ms, fin_launches = gb.prof_read("msm_final")
+    gb.prof_read("", reset=True)
+    from zkgpu.msm_gpu import _result
+
+    result = _result(out)
+    if not result.is_on_curve():
+        raise RuntimeError("bench: MSM result is not on the curve")
+    c_used = window or _default_window(gb, n)
+    entries = signed_digit_count(scalars_host, c_used)
+    mads_per_entry = 11 * 136
+    acc_avg = acc_ms / max(acc_launches, 1)
+    achieved_tmads = entries * mads_per_entry / (acc_avg * 1e-3) / 1e12
+    peak_tmads = _valu_peak(gb)
+    msm_mpts = n / (ms * 1e-3) / 1e6
+    log(f"msm 2^{args.log_msm}: {ms:.3f} ms/step, {msm_mpts:.2f} Mpts/s; accumulate {acc_avg:.3f} ms "
+        f"({achieved_tmads:.2f} Tmad/s of {peak_tmads:.2f}); final {fin_ms / max(fin_launches, 1):.3f} ms")
+
+    extras: dict = {}
+    if not args.no_extras:
+        extras.update(_ntt_and_pipeline(args, gb, BN254_FR, workloads, log))
+    cpu = None if args.no_cpu else _cpu_baseline(points, scalars_host, gb, log)
+
+    line = {
+        "metric": "G1 MSM Mpts/s @2^20, Fr NTT Melem/s @2^22, batched/plain speedup, p99 ms",
+        "value": round(msm_mpts, 4),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (29-bit-limb Montgomery Fq, exact)",
+        "data": "synthetic: P_i = k_i G with seeded k_i, witness-like seeded scalars",
+        "config": {
+            "workload": f"plain BN254 G1 Pippenger MSM, 2^{args.log_msm} device-resident points, witness-like scalars",
+            "msm_points": n,
+            "window_bits_signed": c_used,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "msm_accumulate",
+            "bound": "valu",
+            "achieved": round(achieved_tmads, 4),
+            "peak": round(peak_tmads, 4),
+            "unit": "Tmad/s",
+            "frac": round(achieved_tmads / peak_tmads, 4) if peak_tmads else None,
+            "traffic": None,
+            "avg_launch_ms": round(acc_avg, 4),
+            "units_per_launch": entries,
+            "work_per_unit": "11 Fq mul x 136 32-bit mad (one mixed addition)",
+        },
+        "cpu_baseline": cpu,
+        **extras,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def _default_window(gb, n: int) -> int:
+    import ctypes
+
+    c = ctypes.c_int(0)
+    gb.call("zk_msm_default_window", n, ctypes.byref(c))
+    return int(c.value)
+
+
+def _valu_peak(gb) -> float:
+    """gfx950 v_mad_u64_u32 throughput in T/s, measured on this GPU (zk_valu_peak)."""
+    import ctypes
+
+    out = ctypes.c_double(0)
+    gb.call("zk_valu_peak", ctypes.byref(out))
+    return float(out.value)
+
+
+def _ntt_and_pipeline(args, gb, Fr, workloads, log) -> dict:
+    n = 1 << args.log_ntt
+    log(f"ntt: 2^{args.log_ntt} forward, uniform Fr rows")
+    x = Fr.from_limbs(workloads.uniform_rows(n, 0x5EED03))
+    out = gb.empty((n, 4))
+    from zkgpu.ntt import get_ntt_root
+    from zkgpu.field import FIELD_MODULUS as R
+
+    omega = gb.limbs_of(get_ntt_root(n, R))
+
+    def ntt_step() -> None:
+        gb.call("zk_ntt", 0, x.ptr, out.ptr, args.log_ntt, 0, omega, None, gb.current_stream())
+
+    ntt_step()
+    gb.synchronize()
+    gb.prof_read("ntt_pass", reset=True)
+    ms = timed(ntt_step, args.steps, 1, gb)
+    pass_ms, passes = gb.prof_read("ntt_pass", reset=True)
+    per_pass = pass_ms / max(passes, 1)
+    passes_per_ntt = max(1, round(passes / (args.steps + 1)))
+    melem = n / (ms * 1e-3) / 1e6
+    hbm_gbps = 64.0 * n / (ms * 1e-3) / 1e9
+    pass_gbps = 64.0 * n / (per_pass * 1e-3) / 1e9  # each pass reads and writes every row once
+    log(f"ntt 2^{args.log_ntt}: {ms:.3f} ms, {melem:.1f} Melem/s; pass {per_pass:.3f} ms ({pass_gbps:.0f} GB/s)")
+
+    log("groth16 quotient pipeline at the same size (7 NTTs)")
+    a, b, c = (Fr.from_limbs(workloads.uniform_rows(n, s)) for s in (0x5EED31, 0x5EED32, 0x5EED33))
+    h = gb.empty((n, 4))
+    shift = gb.limbs_of(7)
+
+    def pipe_step() -> None:
+        gb.call("zk_groth16_quotient", a.ptr, b.ptr, c.ptr, h.ptr, args.log_ntt, shift, gb.current_stream())
+
+    pipe_step()
+    pms = timed(pipe_step, max(1, args.steps // 2), 1, gb)
+    log(f"groth16 2^{args.log_ntt}: {pms:.3f} ms")
+    return {
+        "ntt": {
+            "log_n": args.log_ntt,
+            "ms": round(ms, 4),
+            "melem_s": round(melem, 2),
+            "algorithmic_gbps": round(hbm_gbps, 1),
+            "pass_ms": round(per_pass, 4),
+            "passes": passes_per_ntt,
+            "roofline": {
+                "kernel": "ntt_pass",
+                "bound": "hbm",
+                "achieved": round(pass_gbps, 1),
+                "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s",
+                "frac": round(pass_gbps / HBM_PEAK_GBPS, 4),
+                "traffic": None,
+            },
+        },
+        "groth16_quotient": {"log_n": args.log_ntt, "ms": round(pms, 4), "coset_shift": 7},
+    }
+
+
+def _cpu_baseline(points, scalars_host, gb, log) -> dict:
+    """The oracle's Python msm_pippenger on 2^12 points of the same workload, one core."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from zkgpu.field256_gpu import BN254_FQ, _limbs_to_ints
+    from zkref.curve import Point
+    from zkref.msm import msm_pippenger
+
+    sample = 1 << 12
+    coords = _limbs_to_ints(BN254_FQ.to_limbs(points[:sample].reshape(-1, 4)).get())
+    pts = [Point(coords[2 * i], coords[2 * i + 1]) for i in range(sample)]
+    sc = _limbs_to_ints(scalars_host[:sample])
+    log(f"cpu baseline: oracle msm_pippenger on {sample} points (1 core)")
+    t0 = time.perf_counter()
+    msm_pippenger(pts, sc)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(sample / dt / 1e6, 6),
+        "unit": "Mpts/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"first {sample} points and scalars of the 2^20 workload, Python-int Pippenger (w={_ref_w(sample)}), {dt:.1f} s",
+    }
+
+
+def _ref_w(n: int) -> int:
+    return min(range(1, 17), key=lambda w: (-(-254 // w)) * (n + 2 ** (w + 1)))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -56,6 +56,23 @@ int zk_event_destroy(void* event);
 int zk_event_record(void* event, void* stream);
 int zk_event_elapsed_ms(void* start, void* end, float* ms);
 int zk_stream_wait_event(void* stream, void* event);
+/* HIP graphs over a created stream: everything enqueued between begin and end (e.g. plan
+ * launches, which allocate nothing) is recorded once and replayed by zk_graph_launch (no
+ * counterpart in the reference, which launches from Python every time). */
+int zk_graph_begin(void* stream);
+int zk_graph_end(void* stream, void** graph_exec);
+int zk_graph_launch(void* graph_exec, void* stream);
+int zk_graph_destroy(void* graph_exec);
+/* Opt-in per-kernel timing with HIP events on each launch's stream (no counterpart in the
+ * reference, whose benchmarks time whole calls, benchmarks/_harness.py:67-112):
+ * zk_prof_read sums the recorded durations of the kernels named name ("msm_accumulate",
+ * "ntt_pass", ...) and optionally forgets all records. */
+int zk_prof_enable(int on);
+int zk_prof_read(const char* name, double* total_ms, int64_t* count, int reset);
+/* Measured throughput of v_mad_u64_u32 (32x32+64-bit multiply-add, the instruction every
+ * field product is built from) on the current GPU, in 10^12 per second: the VALU roofline
+ * of the field, NTT and MSM kernels (no counterpart in the reference). */
+int zk_valu_peak(double* mads_per_second_t);
 /* Probe kernel: replaces the CuPy probe (gpu_backend.py:94-99). */
 int zk_selftest(void);
 
@@ -89,8 +106,16 @@ int zk_ntt_cache_clear(void);
  * a, b, c, h: n mont rows each (h may alias none of them). */
 int zk_groth16_quotient(const uint64_t* a, const uint64_t* b, const uint64_t* c, uint64_t* h,
                         int log_n, const uint64_t shift[4], void* stream);
-/* Several Fr NTTs of different sizes in one batched launch sequence. Per job j: in[j], out[j]
- * (device addresses), log_n[j], inverse[j]; the roots are the standard ones from generator 5. */
+/* Several Fr NTTs of different sizes as one launch per pass index (every transform's pass p
+ * in the same launch). Per job j: in[j], out[j] (device addresses, in may equal out), log_n[j]
+ * in 1..28, inverse[j] (the inverse includes the n^-1 scale); roots from the generator 5, as
+ * Field256.ntt (field256_gpu.py:347-387). A plan allocates all scratch and uploads the work
+ * tables; launching it allocates nothing (capturable into a HIP graph). */
+int zk_ntt_plan_create(int count, const uint64_t* const* in, uint64_t* const* out, const int32_t* log_n,
+                       const int32_t* inverse, void** plan, void* stream);
+int zk_ntt_plan_launch(void* plan, void* stream);
+int zk_ntt_plan_destroy(void* plan);
+/* Plan, launch and release in one call. */
 int zk_ntt_batch(int count, const uint64_t* const* in, uint64_t* const* out, const int32_t* log_n,
                  const int32_t* inverse, void* stream);
 
@@ -114,12 +139,27 @@ int zk_g1_fixed_base(const uint64_t* k, int64_t n, uint64_t* out_affine, void* s
  * (0 = automatic); chunk_len: bucket entries per thread in the accumulation (0 = automatic). */
 int zk_msm_g1(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window,
               int chunk_len, uint64_t* out_jac, void* stream);
-/* count MSMs in one batched launch sequence; job j uses points_affine[j] (may all point into one
- * shared basis), scalars[j], n[j] (host arrays of device addresses / sizes) and writes
- * out_jac + 12 * j. window[j] may be 0 (automatic). workspace/workspace_bytes: reserved, pass NULL/0. */
-int zk_msm_g1_batch(int count, const uint64_t* const* points_affine, const uint64_t* const* scalars,
-                    const int64_t* n, const int32_t* window, uint64_t* out_jac, void* workspace,
-                    uint64_t* workspace_bytes, void* stream);
+/* The signed-digit window zk_msm_g1 picks for n points when window = 0. */
+int zk_msm_default_window(int64_t n, int* window);
+/* A device-resident basis: n affine mont points converted once into the kernels' internal
+ * layout and kept in HBM (the shared SRS of batched and served MSMs; no counterpart in the
+ * reference, which converts its points on every call, msm_gpu.py:348). */
+int zk_srs_create(const uint64_t* points_affine, int64_t n, void** srs, void* stream);
+int zk_srs_destroy(void* srs);
+/* A batch of count MSMs over prefixes of one SRS: job j sums n[j] <= SRS size points with the
+ * plain scalars at scalars[j] (device address, n[j] rows) and writes its Jacobian mont result to
+ * out_jac + 12 * j. window[j] may be 0 (automatic) or window may be NULL; chunk_len 0 = automatic.
+ * Creating a plan allocates every buffer the batch needs; launching it enqueues a fixed kernel
+ * sequence that allocates nothing, so it can be captured into a HIP graph. The plan reads the
+ * scalar buffers at launch time. Replaces the per-call structure of msm_pippenger_gpu
+ * (msm_gpu.py:336-377) with one sort, one accumulation and one reduction for all jobs. */
+int zk_msm_plan_create(void* srs, int count, const uint64_t* const* scalars, const int64_t* n,
+                       const int32_t* window, int chunk_len, uint64_t* out_jac, void** plan, void* stream);
+int zk_msm_plan_launch(void* plan, void* stream);
+int zk_msm_plan_destroy(void* plan);
+/* Plan, launch and release in one call. */
+int zk_msm_g1_batch(void* srs, int count, const uint64_t* const* scalars, const int64_t* n,
+                    const int32_t* window, uint64_t* out_jac, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,158 @@
+"""
+zkgpu.msm_gpu on the GPU: the MSM equals the oracle's (a unique group element).
+
+Mirrors the reference's tests/test_msm.py GPU cases (windows 1..16, full-size
+scalars, buckets where P and -P cancel or P doubles, reduction mod r, heavy
+buckets, chunked sums with chunk lengths 2, 3 and 32, boolean scalars) and
+adds the benchmark shapes: the witness-like 2^16 and 2^20 MSMs over a basis
+P_i = k_i G, checked against the closed form (sum s_i k_i mod r) G.
+"""
+
+import random
+
+import numpy as np
+import pytest
+
+from zkgpu import gpu_backend, msm_gpu, workloads
+from zkgpu.curve import Point as gPoint
+from zkgpu.field256_gpu import BN254_FQ
+from zkgpu.msm_gpu import msm_g1, msm_pippenger_gpu
+from zkref.curve import G, P, Point
+from zkref.field import R_SCALAR as R, ints_to_limbs, limbs_to_ints
+from zkref.msm import msm_double_and_add, msm_naive, msm_pippenger
+
+pytestmark = pytest.mark.gpu
+
+
+def pippenger(points, scalars, window_size=None):
+    px = np.array([p.x for p in points], dtype=object)
+    py = np.array([p.y for p in points], dtype=object)
+    got = msm_pippenger_gpu(px, py, np.array(scalars, dtype=object), window_size=window_size)
+    return None if got.is_identity() else got.affine()
+
+
+def aff(p):
+    return None if p.is_identity() else p.affine()
+
+
+@pytest.fixture(scope="module")
+def msm_inputs():
+    rng = random.Random(29)
+    points = [G * rng.randint(1, 2**32) for _ in range(30)]
+    scalars = [rng.randint(0, 2**32) for _ in range(30)]
+    return points, scalars, msm_naive(points, scalars)
+
+
+@pytest.fixture(scope="module")
+def large_inputs():
+    rng = random.Random(37)
+    step = G * rng.randrange(1, R)
+    points = [G * rng.randrange(1, R)]
+    for _ in range(399):
+        points.append(points[-1] + step)
+    scalars = [rng.randrange(R) for _ in points]
+    return points, scalars, msm_double_and_add(points, scalars)
+
+
+@pytest.mark.parametrize("w", [None, 1, 3, 8, 11, 16])
+def test_matches_naive(w, msm_inputs):
+    points, scalars, expected = msm_inputs
+    assert pippenger(points, scalars, w) == aff(expected)
+
+
+@pytest.mark.parametrize("w", [3, 5, 8])
+def test_full_size_scalars(w):
+    rng = random.Random(31)
+    points = [G * rng.randint(1, 2**32) for _ in range(12)]
+    scalars = [rng.randrange(R) for _ in range(12)]
+    assert pippenger(points, scalars, w) == aff(msm_double_and_add(points, scalars))
+
+
+def test_bucket_edge_cases():
+    assert pippenger([G, -G], [7, 7]) is None
+    assert pippenger([G, G], [5, 5]) == (G * 10).affine()
+    assert pippenger([G, G], [0, 0]) is None
+    point = G * 3
+    for s in (-5, R, R + 7, 2**300 + 1, R - 1, (R - 1) // 2, (R + 1) // 2):
+        assert pippenger([point], [s]) == aff(point * (s % R))
+
+
+def test_returns_a_point():
+    assert msm_pippenger_gpu([], [], []) == gPoint.identity()
+    res = msm_pippenger_gpu([G.x], [G.y], [R])
+    assert isinstance(res, gPoint) and res.is_identity()
+    assert msm_pippenger_gpu([G.x, G.x], [G.y, G.y], [2, 3]).affine() == (G * 5).affine()
+
+
+@pytest.mark.parametrize("w", [2, 6, 9, 13, 16])
+def test_at_scale(w, large_inputs):
+    points, scalars, expected = large_inputs
+    assert pippenger(points, scalars, w) == aff(expected)
+
+
+def test_one_heavy_bucket_per_window():
+    points = [G * k for k in range(1, 201)]
+    scalar = 2**200 + 12345
+    assert pippenger(points, [scalar] * 200) == (G * (sum(range(1, 201)) * scalar % R)).affine()
+
+
+@pytest.mark.parametrize("chunk_len", [2, 3, 32])
+@pytest.mark.parametrize("kind", ["boolean", "uniform"])
+def test_chunked_sums(kind, chunk_len, large_inputs, monkeypatch):
+    points, scalars, expected = large_inputs
+    if kind == "boolean":
+        rng = random.Random(41)
+        scalars = [rng.randrange(2) for _ in points]
+        expected = msm_double_and_add(points, scalars)
+    monkeypatch.setattr(msm_gpu, "_CHUNK_LEN", chunk_len)
+    assert pippenger(points, scalars, 9) == aff(expected)
+
+
+def test_off_curve_points_are_rejected_on_the_device():
+    with pytest.raises(ValueError, match=r"\(1, 3\) is not on the BN254 curve"):
+        msm_pippenger_gpu([G.x, 1], [G.y, 3], [5, 6])
+
+
+def test_self_test_failure_is_not_cached(monkeypatch):
+    monkeypatch.setattr(msm_gpu, "_compiled", {})
+    real = msm_gpu._result
+    monkeypatch.setattr(msm_gpu, "_result", lambda rows: -real(rows))
+    with pytest.raises(RuntimeError, match="failed their self-test"):
+        msm_gpu.compile_kernels()
+    assert msm_gpu._compiled == {}
+
+
+def basis(n, seed):
+    keys = workloads.srs_keys(n, seed)
+    pts = workloads.srs_points(ints_to_limbs(keys))
+    return keys, pts
+
+
+def test_fixed_base_points_match_the_oracle():
+    keys, pts = basis(64, 7)
+    host = limbs_to_ints(BN254_FQ.to_limbs(pts.reshape(-1, 4)).get())
+    assert [(host[2 * i], host[2 * i + 1]) for i in range(64)] == [(G * k).affine() for k in keys]
+
+
+@pytest.mark.parametrize("w", [None, 10, 12, 13])
+def test_config2_witness_msm_2_16(w):
+    """2^16 points k_i G and the witness-like scalar mix, against the closed form and the oracle's C Pippenger."""
+    from zkref import fast
+
+    n = 1 << 16
+    keys, pts = basis(n, 0x5EED02)
+    scalars = workloads.witness_scalars(n, 0x5EED02)
+    expected = (G * workloads.closed_form_scalar(keys, scalars)).affine()
+    got = msm_g1(pts, gpu_backend.asarray(ints_to_limbs(scalars)), window_size=w)
+    assert got.affine() == expected
+    if w is None:
+        plain = BN254_FQ.to_limbs(pts.reshape(-1, 4)).get().reshape(n, 2, 4)
+        assert fast.msm_pippenger(plain, ints_to_limbs(scalars), 12) == expected
+
+
+def test_msm_2_20_closed_form():
+    n = 1 << 20
+    keys, pts = basis(n, 0x5EED04)
+    scalars = workloads.witness_scalars(n, 0x5EED04)
+    got = msm_g1(pts, gpu_backend.asarray(ints_to_limbs(scalars)))
+    assert got.affine() == (G * workloads.closed_form_scalar(keys, scalars)).affine()

@@ -174,35 +174,62 @@ __global__ void __launch_bounds__(256) k_prep_points(const uint64_t* __restrict_
   store_aff_rec(out + i, load_aff(f, pts + 8 * i));
 }
 
+// Per-MSM parameters of a batch (see MsmPlan).
+struct MsmJob {
+  const uint64_t* scalars;  // (n, 4) plain limbs
+  int64_t n;
+  int64_t block_base;       // first 256-scalar digit block of this job
+  int64_t key_base;         // first bucket key
+  int64_t seg_base;         // first reduction segment
+  int64_t sums_base;        // first per-window partial-sum row
+  uint64_t* out;            // 12 limbs, the Jacobian result
+  int c, windows, log_seg, log_segs;
+};
+
+// Largest j with base(j) <= x for a field of the job table (bases increase).
+template <int64_t MsmJob::*base>
+__device__ __forceinline__ int find_job(const MsmJob* jobs, int njobs, int64_t x) {
+  int lo = 0, hi = njobs;  // jobs[lo].base <= x < jobs[hi].base
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (jobs[mid].*base <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // mode 0: count keys; mode 1: scatter (key, ref) into sorted position via cursors.
+// Each block handles 256 scalars of one job (jobs are padded to whole blocks).
 template <int mode>
-__global__ void __launch_bounds__(256) k_digits(const uint64_t* __restrict__ scalars, int64_t n, int c, int windows,
+__global__ void __launch_bounds__(256) k_digits(const MsmJob* __restrict__ jobs, int njobs,
                                                 uint32_t* __restrict__ counts_or_cursor, uint32_t* __restrict__ skey,
                                                 uint32_t* __restrict__ sref) {
   __shared__ uint32_t hot_cnt[kHot];
   __shared__ uint32_t hot_base[kHot];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < n;
+  const int jb = find_job<&MsmJob::block_base>(jobs, njobs, blockIdx.x);
+  const MsmJob job = jobs[jb];
+  const int64_t i = ((int64_t)blockIdx.x - job.block_base) * 256 + threadIdx.x;
+  const bool active = i < job.n;
   uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (active) load_scalar(scalars + 4 * i, w);
+  if (active) load_scalar(job.scalars + 4 * i, w);
+  const int c = job.c;
   const uint32_t buckets = 1u << (c - 1);
   const int hot = (int)min((uint32_t)kHot, buckets);
   uint32_t carry = 0;
-  for (int win = 0; win < windows; ++win) {
+  for (int win = 0; win < job.windows; ++win) {
     if (threadIdx.x < kHot) hot_cnt[threadIdx.x] = 0;
     __syncthreads();
     int d = active ? signed_digit(w, win, c, carry) : 0;
     uint32_t mag = (uint32_t)(d < 0 ? -d : d);
-    uint32_t key = (uint32_t)win * buckets + (mag - 1);
+    uint32_t key = (uint32_t)job.key_base + (uint32_t)win * buckets + (mag - 1);
     bool is_hot = mag != 0 && (int)mag <= hot;
     uint32_t local = 0;
     if (is_hot) local = atomicAdd(&hot_cnt[mag - 1], 1u);
     if (mode == 0 && mag != 0 && !is_hot) atomicAdd(&counts_or_cursor[key], 1u);
     __syncthreads();
     if (threadIdx.x < hot && hot_cnt[threadIdx.x]) {
-      uint32_t k = (uint32_t)win * buckets + threadIdx.x;
-      uint32_t base = atomicAdd(&counts_or_cursor[k], hot_cnt[threadIdx.x]);
-      hot_base[threadIdx.x] = base;
+      uint32_t k = (uint32_t)job.key_base + (uint32_t)win * buckets + threadIdx.x;
+      hot_base[threadIdx.x] = atomicAdd(&counts_or_cursor[k], hot_cnt[threadIdx.x]);
     }
     __syncthreads();
     if (mode == 1 && mag != 0) {
@@ -322,33 +349,45 @@ __global__ void __launch_bounds__(256) k_combine(const uint32_t* __restrict__ in
   }
 }
 
-// Leaf of the bucket reduction: segment s of window w covers buckets [sL, sL + L)
+// Leaf of the bucket reduction: segment s of a window covers buckets [sL, sL + L)
 // (bucket b holds the digit value b + 1): T = sum (i + 1) B_{sL+i}, R = sum B_{sL+i}.
-__global__ void __launch_bounds__(256) k_reduce_leaf(const JacRec* __restrict__ buckets, int64_t segments, int seg_len,
-                                                     JacRec* __restrict__ tot, JacRec* __restrict__ run) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // global segment index (window-major)
+__global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ jobs, int njobs, int64_t segments,
+                                                     const JacRec* __restrict__ buckets, JacRec* __restrict__ tot,
+                                                     JacRec* __restrict__ run) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= segments) return;
+  const MsmJob& job = jobs[find_job<&MsmJob::seg_base>(jobs, njobs, g)];
+  const int64_t local = g - job.seg_base;
+  const int seg_len = 1 << job.log_seg;
+  const int64_t win = local >> job.log_segs, s = local & ((1LL << job.log_segs) - 1);
+  const JacRec* b = buckets + job.key_base + (win << (job.c - 1)) + s * seg_len;
   const Fq f;
   Jac r = jac_identity(f), t = jac_identity(f);
   for (int i = seg_len - 1; i >= 0; --i) {
-    r = jac_add(f, r, load_jac_rec(buckets + g * seg_len + i));
+    r = jac_add(f, r, load_jac_rec(b + i));
     t = jac_add(f, t, r);
   }
   store_jac_rec(tot + g, t);
   store_jac_rec(run + g, r);
 }
 
-// One workgroup per (window, j): j < log2(S) gives Q_j = sum of run[s] over s with bit j set;
-// j = log2(S) gives sum of tot[s].
-__global__ void __launch_bounds__(256) k_reduce_bits(const JacRec* __restrict__ tot, const JacRec* __restrict__ run,
-                                                     int64_t segs_per_window, int log_segs, JacRec* __restrict__ out) {
+// One workgroup per (job, window, j): j < log_segs gives Q_j = sum of R_s over segments s with
+// bit j set; j = log_segs gives sum of T_s. Output row = the workgroup index.
+__global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ jobs, int njobs,
+                                                     const JacRec* __restrict__ tot, const JacRec* __restrict__ run,
+                                                     JacRec* __restrict__ out) {
   __shared__ JacRec sh[256];
-  const int w = blockIdx.x / (log_segs + 1), j = blockIdx.x % (log_segs + 1);
+  const MsmJob& job = jobs[find_job<&MsmJob::sums_base>(jobs, njobs, blockIdx.x)];
+  const int64_t local = (int64_t)blockIdx.x - job.sums_base;
+  const int per = job.log_segs + 1;
+  const int64_t win = local / per;
+  const int j = (int)(local % per);
+  const int64_t segs = 1LL << job.log_segs;
   const Fq f;
   Jac acc = jac_identity(f);
-  const JacRec* src = (j == log_segs ? tot : run) + (int64_t)w * segs_per_window;
-  for (int64_t s = threadIdx.x; s < segs_per_window; s += 256) {
-    if (j == log_segs || ((s >> j) & 1)) acc = jac_add(f, acc, load_jac_rec(src + s));
+  const JacRec* src = (j == job.log_segs ? tot : run) + job.seg_base + win * segs;
+  for (int64_t s = threadIdx.x; s < segs; s += 256) {
+    if (j == job.log_segs || ((s >> j) & 1)) acc = jac_add(f, acc, load_jac_rec(src + s));
   }
   store_jac_rec(&sh[threadIdx.x], acc);
   __syncthreads();
@@ -363,28 +402,31 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const JacRec* __restrict__ 
   if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
-// Horner over every bit position: window w contributes sum_tot at 2^(cw) and Q_j at 2^(cw + log2 L + j).
-__global__ void k_final(const JacRec* __restrict__ sums, int windows, int c, int log_seg, int log_segs,
-                        uint64_t* __restrict__ out_jac) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Horner over every bit position, one thread per job: window w contributes sum T at 2^(cw)
+// and Q_j at 2^(cw + log2 L + j).
+__global__ void k_final(const MsmJob* __restrict__ jobs, int njobs, const JacRec* __restrict__ sums) {
+  const int jb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jb >= njobs) return;
+  const MsmJob& job = jobs[jb];
   const Fq f;
   Jac acc = jac_identity(f);
-  for (int w = windows - 1; w >= 0; --w) {
-    const JacRec* ws = sums + (int64_t)w * (log_segs + 1);
-    for (int e = c - 1; e >= 0; --e) {
+  const int per = job.log_segs + 1;
+  for (int w = job.windows - 1; w >= 0; --w) {
+    const JacRec* ws = sums + job.sums_base + (int64_t)w * per;
+    for (int e = job.c - 1; e >= 0; --e) {
       acc = jac_dbl(f, acc);
-      const int j = e - log_seg;
-      if (j >= 0 && j < log_segs) acc = jac_add(f, acc, load_jac_rec(ws + j));
-      if (e == 0) acc = jac_add(f, acc, load_jac_rec(ws + log_segs));
+      const int j = e - job.log_seg;
+      if (j >= 0 && j < job.log_segs) acc = jac_add(f, acc, load_jac_rec(ws + j));
+      if (e == 0) acc = jac_add(f, acc, load_jac_rec(ws + job.log_segs));
     }
   }
-  store_jac(f, out_jac, acc);
+  store_jac(f, job.out, acc);
 }
 
 // ---- host ---------------------------------------------------------------------
 
 int default_window(int64_t n) {
-  // Signed digits: about n additions per window plus ~2 * 2^(c-1) for the reduction.
+  // Signed digits: about n additions per window plus ~3 * 2^(c-1) for the reduction.
   int best = 1;
   double best_cost = 1e300;
   for (int c = 1; c <= 20; ++c) {
@@ -398,98 +440,217 @@ int default_window(int64_t n) {
   return best;
 }
 
+// Every buffer one batch of MSMs needs, sized at creation, so that launching
+// allocates nothing (and the launch sequence can be captured into a graph).
+struct MsmPlan {
+  int njobs = 0;
+  int k0 = 16, k1 = 16;
+  std::vector<MsmJob> jobs;
+  std::vector<int64_t> level_max;
+  int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, segments = 0, sums_rows = 0;
+  const AffRec* points = nullptr;
+  // device buffers
+  void* mem = nullptr;
+  MsmJob* d_jobs = nullptr;
+  uint32_t *counts = nullptr, *offsets = nullptr, *m_dev = nullptr, *skey = nullptr, *sref = nullptr, *scan_ws = nullptr;
+  JacRec* buckets = nullptr;
+  uint32_t* slot_key[2] = {nullptr, nullptr};
+  JacRec* slot_val[2] = {nullptr, nullptr};
+  JacRec *tot = nullptr, *run = nullptr, *sums = nullptr;
+};
+
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// jobs: scalars, n, c (0 = default) and out filled in by the caller; the rest is planned here.
+int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, hipStream_t stream, MsmPlan** out) {
+  auto plan = new MsmPlan();
+  plan->points = points;
+  plan->k0 = chunk > 0 ? chunk : 16;
+  if (plan->k0 < 2) {
+    delete plan;
+    return fail("chunk_len must be at least 2");
+  }
+  plan->k1 = std::max(plan->k0, 4);
+  int64_t blocks = 0, keys = 0, entries = 0, segs = 0, sums = 0;
+  for (auto& j : jobs) {
+    if (j.n < 0 || j.n > 0x7fffffff) {
+      delete plan;
+      return fail("MSM size must be in [0, 2^31)");
+    }
+    if (j.c <= 0) j.c = default_window(j.n);
+    if (j.c > 20) {
+      delete plan;
+      return fail("window must be in 1..20");
+    }
+    j.windows = 254 / j.c + 1;
+    j.log_seg = std::min(3, j.c - 1);
+    j.log_segs = (j.c - 1) - j.log_seg;
+    j.block_base = blocks;
+    j.key_base = keys;
+    j.seg_base = segs;
+    j.sums_base = sums;
+    blocks += std::max<int64_t>(1, (j.n + 255) / 256);
+    keys += (int64_t)j.windows << (j.c - 1);
+    entries += (int64_t)j.windows * j.n;
+    segs += (int64_t)j.windows << j.log_segs;
+    sums += (int64_t)j.windows * (j.log_segs + 1);
+  }
+  if (keys >= 0x7fffffff || entries >= 0xffffffffLL) {
+    delete plan;
+    return fail("MSM batch too large for 32-bit bucket keys");
+  }
+  plan->jobs = jobs;
+  plan->njobs = (int)jobs.size();
+  plan->blocks = blocks;
+  plan->nkeys = keys;
+  plan->max_entries = entries;
+  plan->segments = segs;
+  plan->sums_rows = sums;
+  // Accumulation levels (upper bounds): a level with more entries than one thread feeds another.
+  plan->level_max.push_back(entries);
+  while (plan->level_max.back() > (plan->level_max.size() == 1 ? plan->k0 : plan->k1)) {
+    const int k = plan->level_max.size() == 1 ? plan->k0 : plan->k1;
+    plan->level_max.push_back(2 * ((plan->level_max.back() + k - 1) / k));
+    if (plan->level_max.size() > 64) {
+      delete plan;
+      return fail("internal: MSM level plan did not converge");
+    }
+  }
+  int64_t max_slots = 2;
+  for (size_t l = 1; l < plan->level_max.size(); ++l) max_slots = std::max(max_slots, plan->level_max[l]);
+  plan->max_slots = max_slots;
+  // one allocation for everything
+  const size_t e1 = (size_t)std::max<int64_t>(entries, 1);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t at = off;
+    off += align_up(bytes);
+    return at;
+  };
+  const size_t o_jobs = take(sizeof(MsmJob) * jobs.size());
+  const size_t o_counts = take(4 * (size_t)(keys + 1));
+  const size_t o_offsets = take(4 * (size_t)(keys + 1));
+  const size_t o_m = take(16);
+  const size_t o_skey = take(4 * e1);
+  const size_t o_sref = take(4 * e1);
+  const size_t o_scan = take(4 * scan_workspace_words(keys));
+  const size_t o_buckets = take(sizeof(JacRec) * (size_t)keys);
+  const size_t o_sk0 = take(4 * (size_t)max_slots), o_sk1 = take(4 * (size_t)max_slots);
+  const size_t o_sv0 = take(sizeof(JacRec) * (size_t)max_slots), o_sv1 = take(sizeof(JacRec) * (size_t)max_slots);
+  const size_t o_tot = take(sizeof(JacRec) * (size_t)segs), o_run = take(sizeof(JacRec) * (size_t)segs);
+  const size_t o_sums = take(sizeof(JacRec) * (size_t)sums);
+  plan->mem = dev_alloc(off, stream);
+  if (!plan->mem) {
+    delete plan;
+    return 1;
+  }
+  char* base = static_cast<char*>(plan->mem);
+  plan->d_jobs = reinterpret_cast<MsmJob*>(base + o_jobs);
+  plan->counts = reinterpret_cast<uint32_t*>(base + o_counts);
+  plan->offsets = reinterpret_cast<uint32_t*>(base + o_offsets);
+  plan->m_dev = reinterpret_cast<uint32_t*>(base + o_m);
+  plan->skey = reinterpret_cast<uint32_t*>(base + o_skey);
+  plan->sref = reinterpret_cast<uint32_t*>(base + o_sref);
+  plan->scan_ws = reinterpret_cast<uint32_t*>(base + o_scan);
+  plan->buckets = reinterpret_cast<JacRec*>(base + o_buckets);
+  plan->slot_key[0] = reinterpret_cast<uint32_t*>(base + o_sk0);
+  plan->slot_key[1] = reinterpret_cast<uint32_t*>(base + o_sk1);
+  plan->slot_val[0] = reinterpret_cast<JacRec*>(base + o_sv0);
+  plan->slot_val[1] = reinterpret_cast<JacRec*>(base + o_sv1);
+  plan->tot = reinterpret_cast<JacRec*>(base + o_tot);
+  plan->run = reinterpret_cast<JacRec*>(base + o_run);
+  plan->sums = reinterpret_cast<JacRec*>(base + o_sums);
+  hipError_t e = hipMemcpyAsync(plan->d_jobs, plan->jobs.data(), sizeof(MsmJob) * jobs.size(), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    dev_free(plan->mem);
+    delete plan;
+    return fail_hip(e, "MSM plan upload");
+  }
+  *out = plan;
+  return 0;
+}
+
+void msm_plan_destroy(MsmPlan* plan) {
+  if (!plan) return;
+  dev_free(plan->mem);
+  delete plan;
+}
+
+// Enqueues the whole batch; allocates nothing.
+int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
+  if (plan->njobs == 0) return 0;
+  const int nj = plan->njobs;
+  hipLaunchKernelGGL(k_fill_zero<uint32_t>, dim3(grid_for(plan->nkeys, 256)), dim3(256), 0, stream, plan->counts, plan->nkeys);
+  hipLaunchKernelGGL(k_fill_zero<uint32_t>, dim3(grid_for(plan->nkeys * 28, 256)), dim3(256), 0, stream,
+                     reinterpret_cast<uint32_t*>(plan->buckets), plan->nkeys * 28);
+  ZK_LAUNCH_CHECK("msm clear");
+  hipLaunchKernelGGL(k_digits<0>, dim3((unsigned)plan->blocks), dim3(256), 0, stream, plan->d_jobs, nj, plan->counts,
+                     (uint32_t*)nullptr, (uint32_t*)nullptr);
+  ZK_LAUNCH_CHECK("msm count");
+  if (int rc = scan_u32_ws(plan->counts, plan->offsets, plan->nkeys, plan->m_dev, plan->scan_ws, stream)) return rc;
+  {
+    Prof prof("msm_scatter", stream);
+    hipLaunchKernelGGL(k_digits<1>, dim3((unsigned)plan->blocks), dim3(256), 0, stream, plan->d_jobs, nj, plan->offsets,
+                       plan->skey, plan->sref);
+    ZK_LAUNCH_CHECK("msm scatter");
+  }
+  const int64_t threads0 = (plan->max_entries + plan->k0 - 1) / plan->k0;
+  if (threads0 > 0) {
+    Prof prof("msm_accumulate", stream);
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(threads0, 256)), dim3(256), 0, stream, plan->skey, plan->sref,
+                       plan->m_dev, plan->k0, plan->points, plan->buckets, plan->slot_key[0], plan->slot_val[0]);
+    ZK_LAUNCH_CHECK("msm accumulate");
+  }
+  int cur = 0;
+  for (size_t l = 1; l < plan->level_max.size(); ++l) {
+    const int64_t threads = (plan->level_max[l] + plan->k1 - 1) / plan->k1;
+    hipLaunchKernelGGL(k_combine, dim3(grid_for(threads, 256)), dim3(256), 0, stream, plan->slot_key[cur], plan->slot_val[cur],
+                       plan->m_dev, (int)l, plan->k0, plan->k1, plan->buckets, plan->slot_key[cur ^ 1], plan->slot_val[cur ^ 1]);
+    ZK_LAUNCH_CHECK("msm combine");
+    cur ^= 1;
+  }
+  {
+    Prof prof("msm_reduce", stream);
+    hipLaunchKernelGGL(k_reduce_leaf, dim3(grid_for(plan->segments, 256)), dim3(256), 0, stream, plan->d_jobs, nj,
+                       plan->segments, plan->buckets, plan->tot, plan->run);
+    ZK_LAUNCH_CHECK("msm reduce leaf");
+    hipLaunchKernelGGL(k_reduce_bits, dim3((unsigned)plan->sums_rows), dim3(256), 0, stream, plan->d_jobs, nj, plan->tot,
+                       plan->run, plan->sums);
+    ZK_LAUNCH_CHECK("msm reduce bits");
+  }
+  Prof prof("msm_final", stream);
+  hipLaunchKernelGGL(k_final, dim3(grid_for(nj, 64)), dim3(64), 0, stream, plan->d_jobs, nj, plan->sums);
+  ZK_LAUNCH_CHECK("msm final");
+  return 0;
+}
+
+// Points (n, 2, 4) 256-form rows -> packed 261-form records.
+int convert_points(const uint64_t* points, int64_t n, AffRec* out, hipStream_t stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_prep_points, dim3(grid_for(n, 256)), dim3(256), 0, stream, points, n, out);
+  ZK_LAUNCH_CHECK("msm prep");
+  return 0;
+}
+
 int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, int k0, uint64_t* out_jac,
            hipStream_t stream) {
   if (n < 0 || n > 0x7fffffff) return fail("MSM size must be in [0, 2^31)");
-  if (c <= 0) c = default_window(n);
-  if (c < 1 || c > 20) return fail("window must be in 1..20");
-  if (k0 <= 0) k0 = 16;
-  if (k0 < 2) return fail("chunk_len must be at least 2");
-  const int k1 = std::max(k0, 4);
-  const int windows = 254 / c + 1;
-  const int64_t buckets_per_window = 1LL << (c - 1);
-  const int64_t nkeys = (int64_t)windows * buckets_per_window;
-  const int64_t max_entries = (int64_t)windows * n;
+  if (c < 0 || c > 20) return fail("window must be in 1..20");
   Scratch sc(stream);
-
-  // Level plan (upper bounds) and sizes
-  // m_l upper bounds: a level with more entries than one thread takes feeds another level.
-  std::vector<int64_t> level_max;
-  level_max.push_back(max_entries);
-  while (level_max.back() > (level_max.size() == 1 ? k0 : k1)) {
-    const int k = level_max.size() == 1 ? k0 : k1;
-    level_max.push_back(2 * ((level_max.back() + k - 1) / k));
-    if (level_max.size() > 64) return fail("internal: MSM level plan did not converge");
-  }
-  int64_t max_slots = 2;
-  for (size_t l = 1; l < level_max.size(); ++l) max_slots = std::max(max_slots, level_max[l]);
-
   AffRec* pts = sc.get<AffRec>((size_t)std::max<int64_t>(n, 1));
-  uint32_t* counts = sc.get<uint32_t>((size_t)nkeys + 1);
-  uint32_t* offsets = sc.get<uint32_t>((size_t)nkeys + 1);
-  uint32_t* m_dev = sc.get<uint32_t>(1);
-  uint32_t* skey = sc.get<uint32_t>((size_t)std::max<int64_t>(max_entries, 1));
-  uint32_t* sref = sc.get<uint32_t>((size_t)std::max<int64_t>(max_entries, 1));
-  JacRec* buckets = sc.get<JacRec>((size_t)nkeys);
-  uint32_t* slot_key[2] = {sc.get<uint32_t>((size_t)max_slots), sc.get<uint32_t>((size_t)max_slots)};
-  JacRec* slot_val[2] = {sc.get<JacRec>((size_t)max_slots), sc.get<JacRec>((size_t)max_slots)};
-  if (!pts || !counts || !offsets || !m_dev || !skey || !sref || !buckets || !slot_key[0] || !slot_key[1] || !slot_val[0] ||
-      !slot_val[1])
-    return 1;
-
-  const unsigned nb = grid_for(std::max<int64_t>(n, 1), 256);
-  if (n > 0) {
-    hipLaunchKernelGGL(k_prep_points, dim3(nb), dim3(256), 0, stream, points, n, pts);
-    ZK_LAUNCH_CHECK("msm prep");
-  }
-  hipLaunchKernelGGL(k_fill_zero<uint32_t>, dim3(grid_for(nkeys, 256)), dim3(256), 0, stream, counts, nkeys);
-  hipLaunchKernelGGL(k_fill_zero<uint32_t>, dim3(grid_for(nkeys * 28, 256)), dim3(256), 0, stream,
-                     reinterpret_cast<uint32_t*>(buckets), nkeys * 28);
-  ZK_LAUNCH_CHECK("msm clear");
-  if (n > 0) {
-    hipLaunchKernelGGL(k_digits<0>, dim3(nb), dim3(256), 0, stream, scalars, n, c, windows, counts, nullptr, nullptr);
-    ZK_LAUNCH_CHECK("msm count");
-  }
-  if (int rc = scan_u32(counts, offsets, nkeys, m_dev, stream, sc)) return rc;
-  if (n > 0) {
-    hipLaunchKernelGGL(k_digits<1>, dim3(nb), dim3(256), 0, stream, scalars, n, c, windows, offsets, skey, sref);
-    ZK_LAUNCH_CHECK("msm scatter");
-  }
-  // Level 0 and the combine levels
-  {
-    const int64_t threads0 = (max_entries + k0 - 1) / k0;
-    if (threads0 > 0) {
-      hipLaunchKernelGGL(k_accumulate, dim3(grid_for(threads0, 256)), dim3(256), 0, stream, skey, sref, m_dev, k0, pts,
-                         buckets, slot_key[0], slot_val[0]);
-      ZK_LAUNCH_CHECK("msm accumulate");
-    }
-    int cur = 0;
-    for (size_t l = 1; l < level_max.size(); ++l) {
-      const int64_t threads = (level_max[l] + k1 - 1) / k1;
-      hipLaunchKernelGGL(k_combine, dim3(grid_for(threads, 256)), dim3(256), 0, stream, slot_key[cur], slot_val[cur], m_dev,
-                         (int)l, k0, k1, buckets, slot_key[cur ^ 1], slot_val[cur ^ 1]);
-      ZK_LAUNCH_CHECK("msm combine");
-      cur ^= 1;
-    }
-  }
-  // Bucket reduction and window combination
-  const int log_seg = std::min(3, c - 1);
-  const int seg_len = 1 << log_seg;
-  const int log_segs = (c - 1) - log_seg;
-  const int64_t segs_per_window = 1LL << log_segs;
-  const int64_t segments = (int64_t)windows * segs_per_window;
-  JacRec* tot = sc.get<JacRec>((size_t)segments);
-  JacRec* run = sc.get<JacRec>((size_t)segments);
-  JacRec* sums = sc.get<JacRec>((size_t)windows * (log_segs + 1));
-  if (!tot || !run || !sums) return 1;
-  hipLaunchKernelGGL(k_reduce_leaf, dim3(grid_for(segments, 256)), dim3(256), 0, stream, buckets, segments, seg_len, tot, run);
-  ZK_LAUNCH_CHECK("msm reduce leaf");
-  hipLaunchKernelGGL(k_reduce_bits, dim3((unsigned)(windows * (log_segs + 1))), dim3(256), 0, stream, tot, run,
-                     segs_per_window, log_segs, sums);
-  ZK_LAUNCH_CHECK("msm reduce bits");
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(64), 0, stream, sums, windows, c, log_seg, log_segs, out_jac);
-  ZK_LAUNCH_CHECK("msm final");
-  return 0;
+  if (!pts) return 1;
+  if (int rc = convert_points(points, n, pts, stream)) return rc;
+  MsmJob job{};
+  job.scalars = scalars;
+  job.n = n;
+  job.c = c;
+  job.out = out_jac;
+  MsmPlan* plan = nullptr;
+  if (int rc = msm_plan_create(pts, {job}, k0, stream, &plan)) return rc;
+  int rc = msm_plan_launch(plan, stream);
+  msm_plan_destroy(plan);  // blocks return to this stream's cache: later work on it is ordered after the launch
+  return rc;
 }
 
 }  // namespace zk
@@ -498,9 +659,76 @@ using namespace zk;
 
 extern "C" {
 
+int zk_msm_default_window(int64_t n, int* window) {
+  *window = default_window(n);
+  return 0;
+}
+
 int zk_msm_g1(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window, int chunk_len,
               uint64_t* out_jac, void* stream) {
   return msm_g1(points_affine, scalars, n, window, chunk_len, out_jac, (hipStream_t)stream);
+}
+
+struct ZkSrs {
+  AffRec* pts;
+  int64_t n;
+};
+
+int zk_srs_create(const uint64_t* points_affine, int64_t n, void** srs, void* stream) {
+  if (n <= 0 || n > 0x7fffffff) return fail("SRS size must be in [1, 2^31)");
+  AffRec* p = nullptr;
+  ZK_HIP(hipMalloc(&p, sizeof(AffRec) * (size_t)n));
+  if (int rc = convert_points(points_affine, n, p, (hipStream_t)stream)) {
+    (void)hipFree(p);
+    return rc;
+  }
+  ZK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  *srs = new ZkSrs{p, n};
+  return 0;
+}
+
+int zk_srs_destroy(void* srs) {
+  if (!srs) return 0;
+  auto* s = static_cast<ZkSrs*>(srs);
+  ZK_HIP(hipDeviceSynchronize());
+  (void)hipFree(s->pts);
+  delete s;
+  return 0;
+}
+
+int zk_msm_plan_create(void* srs, int count, const uint64_t* const* scalars, const int64_t* n, const int32_t* window,
+                       int chunk_len, uint64_t* out_jac, void** plan, void* stream) {
+  auto* s = static_cast<ZkSrs*>(srs);
+  if (!s) return fail("an SRS handle is required");
+  std::vector<MsmJob> jobs((size_t)std::max(count, 0));
+  for (int j = 0; j < count; ++j) {
+    if (n[j] > s->n) return fail("MSM size exceeds the SRS");
+    jobs[(size_t)j] = MsmJob{};
+    jobs[(size_t)j].scalars = scalars[j];
+    jobs[(size_t)j].n = n[j];
+    jobs[(size_t)j].c = window ? window[j] : 0;
+    jobs[(size_t)j].out = out_jac + 12 * j;
+  }
+  MsmPlan* p = nullptr;
+  if (int rc = msm_plan_create(s->pts, jobs, chunk_len, (hipStream_t)stream, &p)) return rc;
+  *plan = p;
+  return 0;
+}
+
+int zk_msm_plan_launch(void* plan, void* stream) { return msm_plan_launch(static_cast<MsmPlan*>(plan), (hipStream_t)stream); }
+
+int zk_msm_plan_destroy(void* plan) {
+  msm_plan_destroy(static_cast<MsmPlan*>(plan));
+  return 0;
+}
+
+int zk_msm_g1_batch(void* srs, int count, const uint64_t* const* scalars, const int64_t* n, const int32_t* window,
+                    uint64_t* out_jac, void* stream) {
+  void* plan = nullptr;
+  if (int rc = zk_msm_plan_create(srs, count, scalars, n, window, 0, out_jac, &plan, stream)) return rc;
+  int rc = zk_msm_plan_launch(plan, stream);
+  zk_msm_plan_destroy(plan);
+  return rc;
 }
 
 }  // extern "C"

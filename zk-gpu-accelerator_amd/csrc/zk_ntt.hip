@@ -37,6 +37,8 @@
 
 namespace zk {
 
+void fr_roots(int log_n, U256* omega, U256* omega_inv);  // zk_pipeline.hip
+
 constexpr int kTile = 1024;  // rows per workgroup tile
 constexpr int kThreads = 256;
 
@@ -75,32 +77,41 @@ struct PassGeom {
   }
 };
 
+// Everything one workgroup of a pass needs (per transform in a batch).
+struct NttPassDesc {
+  const uint64_t* src;
+  uint64_t* dst;
+  const uint64_t* tw;
+  const uint64_t* pre;
+  const uint64_t* post;
+  int post_is_table;
+  PassGeom geo;
+  QuotientArgs q;
+  int64_t tile_base;  // first workgroup of this transform in a batched launch
+};
+
 template <class F>
-__global__ void __launch_bounds__(kThreads) k_ntt_pass(F f, const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
-                                                       const uint64_t* __restrict__ tw, PassGeom geo,
-                                                       const uint64_t* __restrict__ pre, const uint64_t* __restrict__ post,
-                                                       int post_is_table, QuotientArgs q) {
-  __shared__ uint32_t lds[NL * kTile];
+__device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64_t wg, uint32_t* lds) {
+  const PassGeom geo = d.geo;
   const int C = 1 << geo.log_c;
   const int T = C << geo.k;
   const int64_t n = 1LL << geo.log_n;
-  const int64_t wg = blockIdx.x;
   const int log_s = geo.log_n - geo.s0 - geo.k;
 
   for (int e = threadIdx.x; e < T; e += kThreads) {
     const int g = e & (C - 1), m = e >> geo.log_c;
     const int64_t i = geo.row(wg, g, m);
     Fe x;
-    if (q.a) {
+    if (d.q.a) {
       // Groth16 numerator on the coset: (A*B' - C) * z^-1, with B' carrying an extra 2^5
       // so that mont(A, B') is in the 256-form.
-      Fe av = load_row(q.a + 4 * i), bv = load_row(q.b + 4 * i), cv = load_row(q.c + 4 * i);
-      Fe zi = load_row(q.z_inv);
+      Fe av = load_row(d.q.a + 4 * i), bv = load_row(d.q.b + 4 * i), cv = load_row(d.q.c + 4 * i);
+      Fe zi = load_row(d.q.z_inv);
       x = mont(f, fsub(f, mont(f, av, bv), cv), zi);
     } else {
-      x = load_row(src + 4 * i);
+      x = load_row(d.src + 4 * i);
     }
-    if (pre) x = mont(f, x, load_row(pre + 4 * i));
+    if (d.pre) x = mont(f, x, load_row(d.pre + 4 * i));
     lds_put(lds, e, x);
   }
   __syncthreads();
@@ -108,7 +119,7 @@ __global__ void __launch_bounds__(kThreads) k_ntt_pass(F f, const uint64_t* __re
   for (int t = 0; t < geo.k; ++t) {
     const int hb = geo.k - 1 - t;  // bit of m that this stage pairs
     const int s = geo.s0 + t;
-    const uint64_t* tws = tw + 4 * (n - (n >> s));  // stage s table: n >> (s + 1) entries
+    const uint64_t* tws = d.tw + 4 * (n - (n >> s));  // stage s table: n >> (s + 1) entries
     for (int b = threadIdx.x; b < (T >> 1); b += kThreads) {
       const int g = b & (C - 1), qv = b >> geo.log_c;
       const int mlow = qv & ((1 << hb) - 1);
@@ -130,9 +141,30 @@ __global__ void __launch_bounds__(kThreads) k_ntt_pass(F f, const uint64_t* __re
     const int64_t i = geo.row(wg, g, m);
     Fe x = lds_get(lds, e);
     const int64_t o = geo.last ? bitrev(i, geo.log_n) : i;
-    if (post) x = mont(f, x, load_row(post + (post_is_table ? 4 * o : 0)));
-    store_row(dst + 4 * o, x);
+    if (d.post) x = mont(f, x, load_row(d.post + (d.post_is_table ? 4 * o : 0)));
+    store_row(d.dst + 4 * o, x);
   }
+}
+
+template <class F>
+__global__ void __launch_bounds__(kThreads) k_ntt_pass(F f, NttPassDesc d) {
+  __shared__ uint32_t lds[NL * kTile];
+  ntt_tile(f, d, blockIdx.x, lds);
+}
+
+// A pass of several transforms at once: workgroup -> transform by binary search over tile_base.
+template <class F>
+__global__ void __launch_bounds__(kThreads) k_ntt_pass_multi(F f, const NttPassDesc* __restrict__ ds, int nd) {
+  __shared__ uint32_t lds[NL * kTile];
+  int lo = 0, hi = nd;
+  const int64_t wg = blockIdx.x;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (ds[mid].tile_base <= wg) lo = mid;
+    else hi = mid;
+  }
+  const NttPassDesc d = ds[lo];
+  ntt_tile(f, d, wg - d.tile_base, lds);
 }
 
 // Table expansion: out[j] = seed_lo[e & mask] * seed_hi[e >> h] for e = j << shift, in the 261-form.
@@ -174,8 +206,7 @@ struct Table {
 
 std::mutex g_tab_mu;
 std::map<TableKey, Table> g_tables;
-std::list<TableKey> g_lru;
-constexpr size_t kMaxTables = 48;
+std::list<TableKey> g_lru;  // most recently used first (tables are never evicted: plans keep raw pointers)
 
 bool field_modulus(int field_id, U256* p) {
   static const uint64_t r[4] = {0x43e1f593f0000001ULL, 0x2833e84879b97091ULL, 0xb85045b68181585dULL, 0x30644e72e131a029ULL};
@@ -240,14 +271,6 @@ int get_table(const TableKey& key, hipStream_t stream, const uint64_t** out) {
   const int64_t n = 1LL << key.log_n;
   size_t rows = key.kind == 0 ? (size_t)(n > 1 ? n - 1 : 1) : key.kind >= 3 ? 1 : (size_t)n;
   uint64_t* buf = nullptr;
-  if (g_tables.size() >= kMaxTables) {
-    // Evict the least recently used table once no queued work can still read it.
-    ZK_HIP(hipDeviceSynchronize());
-    TableKey victim = g_lru.back();
-    g_lru.pop_back();
-    (void)hipFree(g_tables[victim].ptr);
-    g_tables.erase(victim);
-  }
   ZK_HIP(hipMalloc(&buf, rows * 32));
   int rc = 0;
   if (key.kind == 0) {
@@ -285,6 +308,7 @@ int get_table(const TableKey& key, hipStream_t stream, const uint64_t** out) {
 struct PassPlan {
   int s0, k, log_c;
 };
+
 
 std::vector<PassPlan> plan_passes(int log_n) {
   std::vector<PassPlan> out;
@@ -338,30 +362,157 @@ int const_row(int field_id, const uint64_t value[4], hipStream_t stream, const u
   return get_table(key, stream, out);
 }
 
+// Descriptors of every pass of one transform (tmp: n rows of scratch when there are several passes).
+std::vector<NttPassDesc> pass_descs(const uint64_t* in, uint64_t* out, uint64_t* tmp, int log_n, const uint64_t* tw,
+                                    const uint64_t* pre, const uint64_t* post, bool post_table, const QuotientArgs& q) {
+  std::vector<PassPlan> plan = plan_passes(log_n);
+  std::vector<NttPassDesc> ds;
+  for (size_t p = 0; p < plan.size(); ++p) {
+    const bool first = p == 0, last = p + 1 == plan.size();
+    NttPassDesc d{};
+    d.geo = PassGeom{log_n, plan[p].s0, plan[p].k, plan[p].log_c, last ? 1 : 0};
+    d.src = first ? in : tmp;
+    d.dst = last ? out : tmp;
+    d.tw = tw;
+    d.pre = first ? pre : nullptr;
+    d.post = last ? post : nullptr;
+    d.post_is_table = post_table ? 1 : 0;
+    d.q = first ? q : QuotientArgs{};
+    ds.push_back(d);
+  }
+  return ds;
+}
+
+int64_t pass_tiles(const NttPassDesc& d) { return 1LL << (d.geo.log_n - d.geo.k - d.geo.log_c); }
+
 int ntt_run(int field_id, const uint64_t* in, uint64_t* out, int log_n, const uint64_t* tw, const uint64_t* pre,
             const uint64_t* post, bool post_table, const QuotientArgs& q, hipStream_t stream) {
   const int64_t n = 1LL << log_n;
-  std::vector<PassPlan> plan = plan_passes(log_n);
   Scratch sc(stream);
   uint64_t* tmp = nullptr;
-  if (plan.size() > 1) {
+  if (plan_passes(log_n).size() > 1) {
     tmp = sc.get<uint64_t>((size_t)n * 4);
     if (!tmp) return 1;
   }
+  std::vector<NttPassDesc> ds = pass_descs(in, out, tmp, log_n, tw, pre, post, post_table, q);
   return with_field(field_id, [&](auto f) {
-    for (size_t p = 0; p < plan.size(); ++p) {
-      const bool first = p == 0, last = p + 1 == plan.size();
-      PassGeom geo{log_n, plan[p].s0, plan[p].k, plan[p].log_c, last ? 1 : 0};
-      const uint64_t* src = first ? in : tmp;
-      uint64_t* dst = last ? out : tmp;
-      const int64_t tiles = n >> (plan[p].k + plan[p].log_c);
-      QuotientArgs qa = first ? q : QuotientArgs{};
-      hipLaunchKernelGGL((k_ntt_pass<decltype(f)>), dim3((unsigned)tiles), dim3(kThreads), 0, stream, f, src, dst, tw, geo,
-                         first ? pre : nullptr, last ? post : nullptr, post_table ? 1 : 0, qa);
+    for (const NttPassDesc& d : ds) {
+      Prof prof("ntt_pass", stream);
+      hipLaunchKernelGGL((k_ntt_pass<decltype(f)>), dim3((unsigned)pass_tiles(d)), dim3(kThreads), 0, stream, f, d);
       ZK_LAUNCH_CHECK("ntt pass");
     }
     return 0;
   });
+}
+
+// ---- batched transforms ---------------------------------------------------------
+
+struct NttPlan {
+  std::vector<std::vector<NttPassDesc>> passes;  // by pass index, all transforms with that pass
+  std::vector<NttPassDesc*> d_passes;
+  std::vector<int64_t> tiles;
+  void* mem = nullptr;
+};
+
+int ntt_plan_create(int count, const uint64_t* const* in, uint64_t* const* out, const int32_t* log_n,
+                    const int32_t* inverse, hipStream_t stream, NttPlan** result) {
+  auto plan = new NttPlan();
+  // scratch for every multi-pass transform plus the device descriptor tables
+  size_t tmp_rows = 0;
+  for (int j = 0; j < count; ++j) {
+    if (log_n[j] < 1 || log_n[j] > 28) {
+      delete plan;
+      return fail("batched NTT sizes must be 2^1..2^28");
+    }
+    if (plan_passes(log_n[j]).size() > 1) tmp_rows += (size_t)1 << log_n[j];
+  }
+  std::vector<std::vector<NttPassDesc>> per_job;
+  for (int j = 0; j < count; ++j) {
+    U256 w, wi;
+    fr_roots(log_n[j], &w, &wi);
+    const uint64_t* tw = nullptr;
+    if (int rc = twiddles(0, log_n[j], inverse[j] ? wi.w : w.w, stream, &tw)) {
+      delete plan;
+      return rc;
+    }
+    const uint64_t* post = nullptr;
+    const uint64_t one[4] = {1, 0, 0, 0};
+    if (inverse[j]) {
+      if (int rc = scale_row(0, log_n[j], one, stream, &post)) {
+        delete plan;
+        return rc;
+      }
+    }
+    per_job.push_back(pass_descs(in[j], out[j], nullptr, log_n[j], tw, nullptr, post, false, QuotientArgs{}));
+  }
+  size_t max_passes = 0;
+  for (auto& v : per_job) max_passes = std::max(max_passes, v.size());
+  plan->passes.resize(max_passes);
+  size_t desc_count = 0;
+  for (auto& v : per_job) desc_count += v.size();
+  const size_t bytes = tmp_rows * 32 + desc_count * sizeof(NttPassDesc) + 256;
+  plan->mem = dev_alloc(bytes, stream);
+  if (!plan->mem) {
+    delete plan;
+    return 1;
+  }
+  uint64_t* tmp = static_cast<uint64_t*>(plan->mem);
+  NttPassDesc* dtab = reinterpret_cast<NttPassDesc*>(static_cast<char*>(plan->mem) + ((tmp_rows * 32 + 255) & ~(size_t)255));
+  size_t tmp_at = 0;
+  for (int j = 0; j < count; ++j) {
+    auto& v = per_job[(size_t)j];
+    if (v.size() > 1) {
+      for (size_t p = 0; p < v.size(); ++p) {
+        if (p > 0) v[p].src = tmp + tmp_at;
+        if (p + 1 < v.size()) v[p].dst = tmp + tmp_at;
+      }
+      tmp_at += ((size_t)1 << log_n[j]) * 4;
+    }
+    for (size_t p = 0; p < v.size(); ++p) plan->passes[p].push_back(v[p]);
+  }
+  size_t at = 0;
+  for (size_t p = 0; p < max_passes; ++p) {
+    int64_t tiles = 0;
+    for (auto& d : plan->passes[p]) {
+      d.tile_base = tiles;
+      tiles += pass_tiles(d);
+    }
+    plan->tiles.push_back(tiles);
+    plan->d_passes.push_back(dtab + at);
+    hipError_t e = hipMemcpyAsync(dtab + at, plan->passes[p].data(), sizeof(NttPassDesc) * plan->passes[p].size(),
+                                  hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) {
+      dev_free(plan->mem);
+      delete plan;
+      return fail_hip(e, "NTT plan upload");
+    }
+    at += plan->passes[p].size();
+  }
+  hipError_t e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    dev_free(plan->mem);
+    delete plan;
+    return fail_hip(e, "NTT plan upload");
+  }
+  *result = plan;
+  return 0;
+}
+
+int ntt_plan_launch(NttPlan* plan, hipStream_t stream) {
+  const FrConst f;
+  for (size_t p = 0; p < plan->passes.size(); ++p) {
+    Prof prof("ntt_pass_multi", stream);
+    hipLaunchKernelGGL((k_ntt_pass_multi<FrConst>), dim3((unsigned)plan->tiles[p]), dim3(kThreads), 0, stream, f,
+                       plan->d_passes[p], (int)plan->passes[p].size());
+    ZK_LAUNCH_CHECK("batched ntt pass");
+  }
+  return 0;
+}
+
+void ntt_plan_destroy(NttPlan* plan) {
+  if (!plan) return;
+  dev_free(plan->mem);
+  delete plan;
 }
 
 }  // namespace zk
@@ -371,6 +522,30 @@ using namespace zk;
 extern "C" {
 
 int zk_ntt_cache_clear(void) { return ntt_cache_clear(); }
+
+int zk_ntt_plan_create(int count, const uint64_t* const* in, uint64_t* const* out, const int32_t* log_n,
+                       const int32_t* inverse, void** plan, void* stream) {
+  NttPlan* p = nullptr;
+  if (int rc = ntt_plan_create(count, in, out, log_n, inverse, (hipStream_t)stream, &p)) return rc;
+  *plan = p;
+  return 0;
+}
+
+int zk_ntt_plan_launch(void* plan, void* stream) { return ntt_plan_launch(static_cast<NttPlan*>(plan), (hipStream_t)stream); }
+
+int zk_ntt_plan_destroy(void* plan) {
+  ntt_plan_destroy(static_cast<NttPlan*>(plan));
+  return 0;
+}
+
+int zk_ntt_batch(int count, const uint64_t* const* in, uint64_t* const* out, const int32_t* log_n, const int32_t* inverse,
+                 void* stream) {
+  void* plan = nullptr;
+  if (int rc = zk_ntt_plan_create(count, in, out, log_n, inverse, &plan, stream)) return rc;
+  int rc = zk_ntt_plan_launch(plan, stream);
+  zk_ntt_plan_destroy(plan);
+  return rc;
+}
 
 int zk_ntt(int field_id, const uint64_t* in, uint64_t* out, int log_n, int inverse, const uint64_t omega[4],
            const uint64_t* coset, void* stream_) {

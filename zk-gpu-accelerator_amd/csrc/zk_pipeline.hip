@@ -79,26 +79,4 @@ int zk_groth16_quotient(const uint64_t* a, const uint64_t* b, const uint64_t* c,
   return groth16_quotient(a, b, c, h, log_n, shift, (hipStream_t)stream);
 }
 
-int zk_ntt_batch(int count, const uint64_t* const* in, uint64_t* const* out, const int32_t* log_n, const int32_t* inverse,
-                 void* stream) {
-  for (int j = 0; j < count; ++j) {
-    U256 w, wi;
-    fr_roots(log_n[j], &w, &wi);
-    if (int rc = zk_ntt(0, in[j], out[j], log_n[j], inverse[j], inverse[j] ? wi.w : w.w, nullptr, stream)) return rc;
-  }
-  return 0;
-}
-
-int zk_msm_g1_batch(int count, const uint64_t* const* points_affine, const uint64_t* const* scalars, const int64_t* n,
-                    const int32_t* window, uint64_t* out_jac, void* workspace, uint64_t* workspace_bytes, void* stream) {
-  (void)workspace;
-  (void)workspace_bytes;
-  for (int j = 0; j < count; ++j) {
-    if (int rc = msm_g1(points_affine[j], scalars[j], n[j], window ? window[j] : 0, 0, out_jac + 12 * j,
-                        (hipStream_t)stream))
-      return rc;
-  }
-  return 0;
-}
-
 }  // extern "C"

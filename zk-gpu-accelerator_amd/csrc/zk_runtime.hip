@@ -112,12 +112,58 @@ void* Scratch::raw(size_t bytes) {
   return p;
 }
 
+// ---- opt-in kernel timing ------------------------------------------------
+struct ProfRecord {
+  std::string name;
+  hipEvent_t start, end;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRecord> g_prof;
+
+Prof::Prof(const char* name, hipStream_t stream) : name_(name), stream_(stream) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_prof_on) return;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream_, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
+  if (hipEventCreate(&start_) != hipSuccess || hipEventRecord(start_, stream_) != hipSuccess) start_ = nullptr;
+}
+
+Prof::~Prof() {
+  if (!start_) return;
+  hipEvent_t end = nullptr;
+  if (hipEventCreate(&end) != hipSuccess || hipEventRecord(end, stream_) != hipSuccess) return;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof.push_back(ProfRecord{name_, start_, end});
+}
+
 static int g_cus = 0;
 int device_cus() { return g_cus > 0 ? g_cus : 256; }
 
 __global__ void probe_kernel(int64_t* x, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = x[i] + 1;
+}
+
+// v_mad_u64_u32 throughput: eight independent 64-bit accumulators per thread,
+// each step one 32x32+64 multiply-add (the instruction every field product here is built from).
+__global__ void __launch_bounds__(256) k_mad_peak(uint32_t* out, int iters, uint32_t seed) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t acc[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) acc[c] = (uint64_t)(seed ^ (t * 2654435761u + c)) << 7;
+  const uint32_t b = seed * 40503u + t;
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[c] = (uint64_t)(uint32_t)acc[c] * b + acc[c];  // multiplicand from the chain
+    }
+  }
+  uint64_t s = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) s ^= acc[c];
+  out[t] = (uint32_t)s ^ (uint32_t)(s >> 32);
 }
 
 }  // namespace zk
@@ -272,6 +318,85 @@ int zk_event_elapsed_ms(void* start, void* end, float* ms) {
 
 int zk_stream_wait_event(void* stream, void* event) {
   ZK_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+  return 0;
+}
+
+int zk_prof_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_on = on != 0;
+  return 0;
+}
+
+int zk_prof_read(const char* name, double* total_ms, int64_t* count, int reset) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  double sum = 0;
+  int64_t c = 0;
+  for (auto& r : g_prof) {
+    if (r.name != name) continue;
+    float ms = 0;
+    ZK_HIP(hipEventSynchronize(r.end));
+    ZK_HIP(hipEventElapsedTime(&ms, r.start, r.end));
+    sum += ms;
+    ++c;
+  }
+  *total_ms = sum;
+  *count = c;
+  if (reset) {
+    for (auto& r : g_prof) {
+      (void)hipEventDestroy(r.start);
+      (void)hipEventDestroy(r.end);
+    }
+    g_prof.clear();
+  }
+  return 0;
+}
+
+int zk_graph_begin(void* stream) {
+  if (!stream) return fail("graph capture needs a created stream, not the null stream");
+  ZK_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+  return 0;
+}
+
+int zk_graph_end(void* stream, void** graph_exec) {
+  hipGraph_t g = nullptr;
+  ZK_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail_hip(e, "hipGraphInstantiate");
+  *graph_exec = (void*)ex;
+  return 0;
+}
+
+int zk_graph_launch(void* graph_exec, void* stream) {
+  ZK_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+  return 0;
+}
+
+int zk_graph_destroy(void* graph_exec) {
+  if (graph_exec) ZK_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+  return 0;
+}
+
+int zk_valu_peak(double* mads_per_second_t) {
+  const int blocks = device_cus() * 8, threads = 256, iters = 2000;
+  uint32_t* out = nullptr;
+  ZK_HIP(hipMalloc(&out, (size_t)blocks * threads * 4));
+  hipEvent_t e0, e1;
+  ZK_HIP(hipEventCreate(&e0));
+  ZK_HIP(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_mad_peak, dim3(blocks), dim3(threads), 0, 0, out, 10, 1u);  // warm up
+  ZK_HIP(hipEventRecord(e0, 0));
+  hipLaunchKernelGGL(k_mad_peak, dim3(blocks), dim3(threads), 0, 0, out, iters, 7u);
+  ZK_HIP(hipEventRecord(e1, 0));
+  ZK_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  ZK_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(out);
+  const double mads = (double)blocks * threads * iters * 16.0 * 8.0;
+  *mads_per_second_t = mads / (ms * 1e-3) / 1e12;
   return 0;
 }
 

@@ -41,6 +41,21 @@ class Scratch {
 
 int device_cus();  // compute units of the current device
 
+// Opt-in kernel timing (zk_prof_enable): a Prof brackets one launch with HIP
+// events on its stream; zk_prof_read sums the durations per name.
+class Prof {
+ public:
+  Prof(const char* name, hipStream_t stream);
+  ~Prof();
+  Prof(const Prof&) = delete;
+  Prof& operator=(const Prof&) = delete;
+
+ private:
+  const char* name_;
+  hipStream_t stream_;
+  hipEvent_t start_ = nullptr;
+};
+
 }  // namespace zk
 
 #define ZK_HIP(expr)                                      \

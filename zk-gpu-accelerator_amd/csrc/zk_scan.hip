@@ -62,20 +62,32 @@ __global__ void k_scan_total(const uint32_t* __restrict__ in, const uint32_t* __
   if (threadIdx.x == 0 && blockIdx.x == 0) *total = n ? out[n - 1] + in[n - 1] : 0u;
 }
 
-int scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, hipStream_t stream, Scratch& sc) {
+size_t scan_workspace_words(int64_t n) {
+  size_t words = 0;
+  while (n > 1) {
+    const int64_t blocks = (n + kScanBlock - 1) / kScanBlock;
+    words += 2 * (size_t)blocks;
+    if (blocks == 1) break;
+    n = blocks;
+  }
+  return words + 2;
+}
+
+int scan_u32_ws(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, uint32_t* ws, hipStream_t stream) {
   if (n <= 0) {
-    if (total) ZK_HIP(hipMemsetAsync(total, 0, 4, stream));
+    if (total) {
+      hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, stream, in, out, (int64_t)0, total);
+      ZK_LAUNCH_CHECK("scan total");
+    }
     return 0;
   }
   const int64_t blocks = (n + kScanBlock - 1) / kScanBlock;
-  uint32_t* sums = sc.get<uint32_t>((size_t)blocks);
-  if (!sums) return 1;
+  uint32_t* sums = ws;
+  uint32_t* offs = ws + blocks;
   hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)blocks), dim3(kScanThreads), 0, stream, in, out, n, sums);
   ZK_LAUNCH_CHECK("scan blocks");
   if (blocks > 1) {
-    uint32_t* offs = sc.get<uint32_t>((size_t)blocks);
-    if (!offs) return 1;
-    if (int rc = scan_u32(sums, offs, blocks, nullptr, stream, sc)) return rc;
+    if (int rc = scan_u32_ws(sums, offs, blocks, nullptr, ws + 2 * blocks, stream)) return rc;
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)blocks), dim3(kScanThreads), 0, stream, out, n, offs);
     ZK_LAUNCH_CHECK("scan add");
   }
@@ -84,6 +96,12 @@ int scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, hipS
     ZK_LAUNCH_CHECK("scan total");
   }
   return 0;
+}
+
+int scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, hipStream_t stream, Scratch& sc) {
+  uint32_t* ws = sc.get<uint32_t>(scan_workspace_words(n));
+  if (!ws) return 1;
+  return scan_u32_ws(in, out, n, total, ws, stream);
 }
 
 }  // namespace zk

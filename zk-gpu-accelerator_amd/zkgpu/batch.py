@@ -1,0 +1,150 @@
+"""
+zkgpu.batch: many MSMs and NTTs of uneven sizes as one launch sequence.
+
+SRS holds a device-resident basis, converted once into the kernels' internal
+layout (the reference converts and uploads its points on every call,
+src/zkgpu/msm_gpu.py:348). MsmBatch plans a set of MSMs over prefixes of an
+SRS: one digit sort, one load-balanced bucket accumulation, one bucket
+reduction and one window combination for all of them together, each MSM with
+its own window width. NttBatch runs several Fr NTTs of different sizes as one
+launch per pass index. Both allocate everything at plan time, so launch()
+enqueues a fixed kernel sequence that can be recorded once into a HIP graph
+(capture()) and replayed.
+
+Window widths default to the autotuned table (zkgpu.autotune) when one exists
+for this GPU, else to the library's cost model.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from collections.abc import Sequence
+from typing import Any
+
+import numpy as np
+
+from zkgpu import gpu_backend as _gb
+from zkgpu.curve import Point, from_jacobian
+from zkgpu.field256_gpu import BN254_FQ, _limbs_to_ints
+from zkgpu.gpu_backend import DeviceArray
+
+
+def _ptr_array(ptrs: Sequence[int]) -> Any:
+    return (ctypes.c_void_p * max(len(ptrs), 1))(*ptrs)
+
+
+class SRS:
+    """A device-resident G1 basis: n affine Montgomery points (finite, on the curve)."""
+
+    def __init__(self, points: Any) -> None:
+        if getattr(points, "ndim", None) != 3 or tuple(points.shape[1:]) != (2, 4) or points.dtype != np.uint64:
+            raise TypeError(f"points must be an (n, 2, 4) uint64 array, got {getattr(points, 'shape', None)}")
+        _gb.require_gpu("SRS")
+        self.points = _gb.asarray(points)
+        self.n = int(points.shape[0])
+        h = ctypes.c_void_p()
+        _gb.call("zk_srs_create", self.points.ptr, self.n, ctypes.byref(h), _gb.current_stream())
+        self.handle = h.value
+
+    @classmethod
+    def from_keys(cls, keys: np.ndarray) -> SRS:
+        """The basis k_i * G for plain (n, 4) scalar rows k_i, generated on the GPU."""
+        from zkgpu.workloads import srs_points
+
+        return cls(srs_points(keys))
+
+    def __del__(self) -> None:
+        if _gb.LIB is not None and getattr(self, "handle", None):
+            _gb.LIB.zk_srs_destroy(self.handle)
+            self.handle = None
+
+
+def _tuned_window(n: int) -> int:
+    try:
+        from zkgpu.autotune import lookup
+
+        return lookup("msm_window", n) or 0
+    except Exception:
+        return 0
+
+
+class MsmBatch:
+    """
+    count MSMs over prefixes of one SRS. scalars[j] is an (n_j, 4) uint64 device
+    array of plain scalars (read at every launch). Results land in self.out,
+    a (count, 3, 4) Jacobian device array; results() converts them to Points.
+    """
+
+    def __init__(self, srs: SRS, scalars: Sequence[DeviceArray], *, windows: Sequence[int] | None = None, chunk_len: int = 0) -> None:
+        _gb.require_gpu("MsmBatch")
+        self.srs = srs
+        self.scalars = [_gb.asarray(s) for s in scalars]
+        self.sizes = [int(s.shape[0]) for s in self.scalars]
+        if any(n > srs.n for n in self.sizes):
+            raise ValueError(f"an MSM is larger than the SRS ({srs.n} points)")
+        count = len(self.scalars)
+        self.windows = list(windows) if windows is not None else [_tuned_window(n) for n in self.sizes]
+        self.out = _gb.empty((max(count, 1), 3, 4))
+        h = ctypes.c_void_p()
+        sizes = (ctypes.c_int64 * max(count, 1))(*self.sizes)
+        wins = (ctypes.c_int32 * max(count, 1))(*self.windows)
+        _gb.call(
+            "zk_msm_plan_create", srs.handle, count, _ptr_array([s.ptr for s in self.scalars]), sizes, wins,
+            chunk_len, self.out.ptr, ctypes.byref(h), _gb.current_stream(),
+        )
+        self.handle = h.value
+        self.count = count
+
+    def launch(self) -> None:
+        _gb.call("zk_msm_plan_launch", self.handle, _gb.current_stream())
+
+    def results(self) -> list[Point]:
+        rows = BN254_FQ.to_limbs(self.out.reshape(-1, 4)).get()
+        flat = _limbs_to_ints(rows)
+        return [from_jacobian((flat[3 * j], flat[3 * j + 1], flat[3 * j + 2])) for j in range(self.count)]
+
+    def __del__(self) -> None:
+        if _gb.LIB is not None and getattr(self, "handle", None):
+            _gb.LIB.zk_msm_plan_destroy(self.handle)
+            self.handle = None
+
+
+class NttBatch:
+    """Several Fr NTTs of different sizes (inputs and outputs (n_j, 4) Montgomery device rows)."""
+
+    def __init__(self, inputs: Sequence[DeviceArray], outputs: Sequence[DeviceArray], inverse: Sequence[bool]) -> None:
+        _gb.require_gpu("NttBatch")
+        if not len(inputs) == len(outputs) == len(inverse):
+            raise ValueError("inputs, outputs and inverse must have the same length")
+        self.inputs = [_gb.asarray(x) for x in inputs]
+        self.outputs = list(outputs)
+        log_n = []
+        for x, y in zip(self.inputs, self.outputs):
+            n = int(x.shape[0])
+            if n < 2 or n & (n - 1) or tuple(y.shape) != (n, 4):
+                raise ValueError("batched NTT sizes must be powers of two >= 2 with matching (n, 4) outputs")
+            log_n.append(n.bit_length() - 1)
+        count = len(self.inputs)
+        h = ctypes.c_void_p()
+        _gb.call(
+            "zk_ntt_plan_create", count, _ptr_array([x.ptr for x in self.inputs]), _ptr_array([y.ptr for y in self.outputs]),
+            (ctypes.c_int32 * max(count, 1))(*log_n), (ctypes.c_int32 * max(count, 1))(*[1 if i else 0 for i in inverse]),
+            ctypes.byref(h), _gb.current_stream(),
+        )
+        self.handle = h.value
+
+    def launch(self) -> None:
+        _gb.call("zk_ntt_plan_launch", self.handle, _gb.current_stream())
+
+    def __del__(self) -> None:
+        if _gb.LIB is not None and getattr(self, "handle", None):
+            _gb.LIB.zk_ntt_plan_destroy(self.handle)
+            self.handle = None
+
+
+def capture(work: Sequence[Any], on: _gb.Stream) -> _gb.Graph:
+    """Record the launch sequence of several plans (MsmBatch / NttBatch) into one HIP graph."""
+    with _gb.Graph.capture(on) as box:
+        for w in work:
+            w.launch()
+    return box[0]

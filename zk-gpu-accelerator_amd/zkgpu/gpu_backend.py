@@ -77,6 +77,14 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
     "zk_stream_wait_event": [_vp, _vp],
     "zk_selftest": [],
+    "zk_graph_begin": [_vp],
+    "zk_graph_end": [_vp, ctypes.POINTER(_vp)],
+    "zk_graph_launch": [_vp, _vp],
+    "zk_graph_destroy": [_vp],
+    "zk_valu_peak": [ctypes.POINTER(ctypes.c_double)],
+    "zk_msm_default_window": [_i64, ctypes.POINTER(_i32)],
+    "zk_prof_enable": [_i32],
+    "zk_prof_read": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), _i64p, _i32],
     "zk_field_register": [_u64p, ctypes.POINTER(_i32)],
     "zk_field_to_mont": [_i32, _vp, _vp, _i64, _vp],
     "zk_field_from_mont": [_i32, _vp, _vp, _i64, _vp],
@@ -92,8 +100,16 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_g1_on_curve": [_vp, _i64, _i64p, _vp],
     "zk_g1_fixed_base": [_vp, _i64, _vp, _vp],
     "zk_msm_g1": [_vp, _vp, _i64, _i32, _i32, _vp, _vp],
-    "zk_msm_g1_batch": [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "zk_srs_create": [_vp, _i64, ctypes.POINTER(_vp), _vp],
+    "zk_srs_destroy": [_vp],
+    "zk_msm_plan_create": [_vp, _i32, _vp, _vp, _vp, _i32, _vp, ctypes.POINTER(_vp), _vp],
+    "zk_msm_plan_launch": [_vp, _vp],
+    "zk_msm_plan_destroy": [_vp],
+    "zk_msm_g1_batch": [_vp, _i32, _vp, _vp, _vp, _vp, _vp],
     "zk_ntt_batch": [_i32, _vp, _vp, _vp, _vp, _vp],
+    "zk_ntt_plan_create": [_i32, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp],
+    "zk_ntt_plan_launch": [_vp, _vp],
+    "zk_ntt_plan_destroy": [_vp],
 }
 STRING_RETURNS = {"zk_last_error": []}
 
@@ -215,6 +231,52 @@ class Event:
     def __del__(self) -> None:
         if LIB is not None and getattr(self, "handle", None):
             LIB.zk_event_destroy(self.handle)
+            self.handle = None
+
+
+def prof_enable(on: bool = True) -> None:
+    """Record HIP events around the library's main kernels (see prof_read)."""
+    call("zk_prof_enable", 1 if on else 0)
+
+
+def prof_read(name: str, reset: bool = False) -> tuple[float, int]:
+    """(total ms, launches) recorded for kernel `name` since the last reset."""
+    total, count = ctypes.c_double(), ctypes.c_int64()
+    call("zk_prof_read", name.encode(), ctypes.byref(total), ctypes.byref(count), 1 if reset else 0)
+    return float(total.value), int(count.value)
+
+
+class Graph:
+    """A HIP graph recorded from a created stream; replay() re-runs the recorded launches."""
+
+    def __init__(self, handle: int, stream_handle: int) -> None:
+        self.handle = handle
+        self.stream = stream_handle
+
+    @classmethod
+    @contextlib.contextmanager
+    def capture(cls, on: Stream) -> Iterator[list]:
+        """with Graph.capture(s) as box: ... launches on s ...; box[0] is the Graph."""
+        box: list = []
+        call("zk_graph_begin", on.handle)
+        ok = False
+        try:
+            with stream(on.handle):
+                yield box
+            ok = True
+        finally:
+            h = ctypes.c_void_p()
+            status = LIB.zk_graph_end(on.handle, ctypes.byref(h))
+            if ok:
+                check(status, "zk_graph_end")
+                box.append(cls(h.value, on.handle))
+
+    def replay(self, stream_handle: int | None = None) -> None:
+        call("zk_graph_launch", self.handle, self.stream if stream_handle is None else stream_handle)
+
+    def __del__(self) -> None:
+        if LIB is not None and getattr(self, "handle", None):
+            LIB.zk_graph_destroy(self.handle)
             self.handle = None
 
 
