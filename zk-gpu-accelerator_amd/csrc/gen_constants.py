@@ -52,6 +52,11 @@ namespace zk {
 """
 
 if __name__ == "__main__":
-    text = HEADER + "\n" + emit("FrConst", R_SCALAR) + "\n\n" + emit("FqConst", P_BASE) + "\n\n}  // namespace zk\n"
+    tail = (
+        "\n\n// The same fields, tagged to use the low-latency Montgomery product (zk_field.hpp).\n"
+        "struct FrIlp : FrConst {\n  static constexpr bool kIlp = true;\n};\n"
+        "struct FqIlp : FqConst {\n  static constexpr bool kIlp = true;\n};\n"
+    )
+    text = HEADER + "\n" + emit("FrConst", R_SCALAR) + "\n\n" + emit("FqConst", P_BASE) + tail + "\n}  // namespace zk\n"
     with open("zk_constants.hpp", "w") as f:
         f.write(text)

@@ -30,4 +30,12 @@ struct FqConst {
   static constexpr uint32_t ninv = 0x04866389u;
 };
 
+// The same fields, tagged to use the low-latency Montgomery product (zk_field.hpp).
+struct FrIlp : FrConst {
+  static constexpr bool kIlp = true;
+};
+struct FqIlp : FqConst {
+  static constexpr bool kIlp = true;
+};
+
 }  // namespace zk

@@ -100,9 +100,9 @@ __device__ __forceinline__ Fe canonical(const F& f, const Fe& a) {
   return r;
 }
 
-// x * y * 2^-261 mod p, below 2p.
+// x * y * 2^-261 mod p, below 2p: one accumulator per column (fewest instructions).
 template <class F>
-__device__ __forceinline__ Fe mont_raw(const F& f, const Fe& a, const Fe& b) {
+__device__ __forceinline__ Fe mont_raw_serial(const F& f, const Fe& a, const Fe& b) {
   uint32_t m[NL];
   Fe r;
   uint64_t acc = 0;
@@ -127,6 +127,62 @@ __device__ __forceinline__ Fe mont_raw(const F& f, const Fe& a, const Fe& b) {
   }
   r.l[NL - 1] = (uint32_t)acc;
   return r;
+}
+
+// The same product with independent partial sums per column: the a*b column sums depend on
+// nothing but the inputs, and the m*p sums take the newest m last, so the chain from one
+// column's m to the next is a few instructions instead of a whole column. About 15% more
+// instructions, several times less latency on one lane (for the serial parts of the MSM).
+template <class F>
+__device__ __forceinline__ Fe mont_raw_ilp(const F& f, const Fe& a, const Fe& b) {
+  uint32_t m[NL];
+  Fe r;
+  uint64_t pa[2 * NL - 1];
+#pragma unroll
+  for (int k = 0; k < 2 * NL - 1; ++k) {
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = (k < NL ? 0 : k - (NL - 1)); i <= (k < NL ? k : NL - 1); ++i) s += (uint64_t)a.l[i] * b.l[k - i];
+    pa[k] = s;
+  }
+  uint64_t carry = 0;
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    uint64_t pm = 0;
+#pragma unroll
+    for (int i = 0; i < k; ++i) pm += (uint64_t)m[i] * f.p[k - i];
+    uint64_t t = pa[k] + carry + pm;
+    m[k] = ((uint32_t)t * f.ninv) & LMASK;
+    t += (uint64_t)m[k] * f.p[0];
+    carry = t >> LB;
+  }
+#pragma unroll
+  for (int k = NL; k < 2 * NL - 1; ++k) {
+    uint64_t pm = 0;
+#pragma unroll
+    for (int i = k - (NL - 1); i < NL; ++i) pm += (uint64_t)m[i] * f.p[k - i];
+    uint64_t t = pa[k] + pm + carry;
+    r.l[k - NL] = (uint32_t)t & LMASK;
+    carry = t >> LB;
+  }
+  r.l[NL - 1] = (uint32_t)carry;
+  return r;
+}
+
+// Field tags may set kIlp to pick the low-latency product.
+template <class F, class = void>
+struct UsesIlp {
+  static constexpr bool value = false;
+};
+template <class F>
+struct UsesIlp<F, decltype((void)F::kIlp)> {
+  static constexpr bool value = F::kIlp;
+};
+
+template <class F>
+__device__ __forceinline__ Fe mont_raw(const F& f, const Fe& a, const Fe& b) {
+  if constexpr (UsesIlp<F>::value) return mont_raw_ilp(f, a, b);
+  else return mont_raw_serial(f, a, b);
 }
 
 // x^2 * 2^-261 mod p, below 2p: the 36 cross products are formed once against
@@ -170,6 +226,7 @@ __device__ __forceinline__ Fe mont(const F& f, const Fe& a, const Fe& b) {
 
 template <class F>
 __device__ __forceinline__ Fe mont_sqr(const F& f, const Fe& a) {
+  if constexpr (UsesIlp<F>::value) return canonical(f, mont_raw_ilp(f, a, a));
   return canonical(f, mont_sqr_raw(f, a));
 }
 

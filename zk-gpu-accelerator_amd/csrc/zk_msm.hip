@@ -44,8 +44,6 @@
 namespace zk {
 
 using Fq = FqConst;
-constexpr uint32_t kInvalid = 0x80000000u;   // slot flag: no value
-constexpr uint32_t kKeyMask = 0x7fffffffu;
 constexpr int kHot = 32;                      // digits 1..kHot are aggregated in LDS
 
 // ---- packed records -------------------------------------------------------
@@ -181,9 +179,11 @@ struct MsmJob {
   int64_t block_base;       // first 256-scalar digit block of this job
   int64_t key_base;         // first bucket key
   int64_t seg_base;         // first reduction segment
-  int64_t sums_base;        // first per-window partial-sum row
+  int64_t sums_base;        // first (window, bit) row of the bucket reduction
+  int64_t part_base;        // first (window, bit, chunk) partial of the bucket reduction
+  int64_t win_base;         // first per-window sum
   uint64_t* out;            // 12 limbs, the Jacobian result
-  int c, windows, log_seg, log_segs;
+  int c, windows, log_seg, log_segs, chunks;
 };
 
 // Largest j with base(j) <= x for a field of the job table (bases increase).
@@ -247,105 +247,146 @@ __global__ void __launch_bounds__(256) k_fill_zero(T* __restrict__ p, int64_t wo
   if (i < words) p[i] = 0;
 }
 
-// Level sizes: m_{l+1} = 2 * ceil(m_l / K_l).
-__device__ __forceinline__ int64_t level_size(uint32_t m0, int level, int k0, int k1) {
-  int64_t m = m0;
-  for (int l = 0; l < level; ++l) {
-    int k = l == 0 ? k0 : k1;
-    m = 2 * ((m + k - 1) / k);
+// Runs of equal bucket keys, summarised per range of sorted entries: the sums of the run that
+// touches the range's first entry (fk) and of the run touching its last entry (lk); runs
+// strictly inside a range are complete and are written to their bucket immediately.
+struct alignas(16) RunSummary {
+  uint32_t fk, lk;
+  uint32_t flags;  // bit 0: any entries; bit 1: a single run (fk == lk covers the whole range)
+  uint32_t pad;
+  JacRec fsum, lsum;
+};
+constexpr uint32_t kValid = 1, kSingle = 2;
+
+// Merges the summary at *l with the one at *r (the range right after it), writing every run that
+// becomes bounded on both sides. At most one point addition.
+__device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, JacRec* __restrict__ buckets) {
+  const FqIlp f;
+  const uint32_t lf = l->flags, rf = r->flags;
+  if (!(rf & kValid)) return;
+  if (!(lf & kValid)) {
+    *l = *r;
+    return;
   }
-  return m;
+  const bool ls = lf & kSingle, rs = rf & kSingle;
+  const uint32_t lk = ls ? l->fk : l->lk;
+  if (lk == r->fk) {
+    // the run crossing the boundary
+    Jac joined = jac_add(f, load_jac_rec(ls ? &l->fsum : &l->lsum), load_jac_rec(&r->fsum));
+    if (ls && rs) {
+      store_jac_rec(&l->fsum, joined);
+    } else if (ls) {
+      store_jac_rec(&l->fsum, joined);
+      l->lk = r->lk;
+      l->lsum = r->lsum;
+      l->flags = kValid;
+    } else if (rs) {
+      store_jac_rec(&l->lsum, joined);
+    } else {
+      store_jac_rec(buckets + lk, joined);  // bounded by both ranges' interiors
+      l->lk = r->lk;
+      l->lsum = r->lsum;
+    }
+    return;
+  }
+  // different keys meet at the boundary
+  if (!ls) buckets[l->lk] = l->lsum;  // l's last run is now closed on both sides
+  if (!rs) buckets[r->fk] = r->fsum;  // r's first run likewise
+  if (ls) {
+    // l's single run stays open on the left
+  }
+  l->lk = rs ? r->fk : r->lk;
+  l->lsum = rs ? r->fsum : r->lsum;
+  l->flags = kValid;
 }
 
-// Level 0: K consecutive sorted entries per thread, mixed additions.
+// Tree over the 256 summaries of a workgroup in LDS; the result is left in sh[0].
+__device__ __forceinline__ void merge_tree(RunSummary* sh, JacRec* __restrict__ buckets) {
+  for (int d = 1; d < 256; d <<= 1) {
+    __syncthreads();
+    if ((threadIdx.x & (2 * d - 1)) == 0) merge_runs(&sh[threadIdx.x], &sh[threadIdx.x + d], buckets);
+  }
+  __syncthreads();
+}
+
+// Writes the open runs of the final summary (they are bounded by the ends of the array).
+__device__ __forceinline__ void close_runs(const RunSummary* sm, JacRec* __restrict__ buckets) {
+  if (!(sm->flags & kValid)) return;
+  buckets[sm->fk] = sm->fsum;
+  if (!(sm->flags & kSingle)) buckets[sm->lk] = sm->lsum;
+}
+
+// Level 0: K consecutive sorted entries per thread (mixed additions), then a workgroup tree.
 __global__ void __launch_bounds__(256) k_accumulate(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sref,
                                                     const uint32_t* __restrict__ m_dev, int k0,
                                                     const AffRec* __restrict__ pts, JacRec* __restrict__ buckets,
-                                                    uint32_t* __restrict__ slot_key, JacRec* __restrict__ slot_val) {
+                                                    RunSummary* __restrict__ out, int last_level) {
+  __shared__ RunSummary sh[256];
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t m = *m_dev;
   const int64_t s = t * k0;
-  if (s >= m) return;
-  const int64_t e = min(s + (int64_t)k0, m);
   const Fq f;
-  const uint32_t first_key = skey[s], last_key = skey[e - 1];
-  const bool head_partial = s > 0 && skey[s - 1] == first_key;
-  const bool tail_partial = e < m && skey[e] == last_key;
-  slot_key[2 * t] = first_key | kInvalid;
-  slot_key[2 * t + 1] = last_key | kInvalid;
-  Jac acc = jac_identity(f);
-  uint32_t cur = first_key;
-  bool first_seg = true;
-  for (int64_t x = s; x <= e; ++x) {
-    const uint32_t key = x < e ? skey[x] : 0xffffffffu;
-    if (key != cur) {
-      // flush segment `cur`
-      const bool is_last = x == e;
-      if (first_seg && head_partial) {
-        slot_key[2 * t] = cur;
-        store_jac_rec(slot_val + 2 * t, acc);
-      } else if (is_last && tail_partial) {
-        slot_key[2 * t + 1] = cur;
-        store_jac_rec(slot_val + 2 * t + 1, acc);
-      } else {
-        store_jac_rec(buckets + cur, acc);
+  RunSummary me;
+  me.flags = 0;
+  me.fk = me.lk = 0;
+  if (s < m) {
+    const int64_t e = min(s + (int64_t)k0, m);
+    Jac acc = jac_identity(f);
+    uint32_t cur = skey[s];
+    bool first = true;
+    me.fk = cur;
+    me.flags = kValid | kSingle;
+    for (int64_t x = s; x <= e; ++x) {
+      const uint32_t key = x < e ? skey[x] : 0xffffffffu;
+      if (key != cur) {
+        const bool is_last = x == e;
+        if (first && is_last) {
+          store_jac_rec(&me.fsum, acc);
+        } else if (first) {
+          store_jac_rec(&me.fsum, acc);
+          me.flags = kValid;
+        } else if (is_last) {
+          me.lk = cur;
+          store_jac_rec(&me.lsum, acc);
+        } else {
+          store_jac_rec(buckets + cur, acc);  // interior run: complete
+        }
+        if (is_last) break;
+        first = false;
+        cur = key;
+        acc = jac_identity(f);
       }
-      if (is_last) break;
-      first_seg = false;
-      cur = key;
-      acc = jac_identity(f);
+      const uint32_t ref = sref[x];
+      Aff q = load_aff_rec(pts + (ref & 0x7fffffffu));
+      if (ref >> 31) q.y = fneg(f, q.y);
+      acc = jac_madd(f, acc, q);
     }
-    const uint32_t ref = sref[x];
-    Aff q = load_aff_rec(pts + (ref & 0x7fffffffu));
-    if (ref >> 31) q.y = fneg(f, q.y);
-    acc = jac_madd(f, acc, q);
+  }
+  sh[threadIdx.x] = me;
+  merge_tree(sh, buckets);
+  if (threadIdx.x == 0) {
+    if (last_level) close_runs(&sh[0], buckets);
+    else out[blockIdx.x] = sh[0];
   }
 }
 
-// Levels >= 1: K consecutive slots per thread, full additions; invalid slots add nothing.
-__global__ void __launch_bounds__(256) k_combine(const uint32_t* __restrict__ in_key, const JacRec* __restrict__ in_val,
-                                                 const uint32_t* __restrict__ m_dev, int level, int k0, int k1,
-                                                 JacRec* __restrict__ buckets, uint32_t* __restrict__ out_key,
-                                                 JacRec* __restrict__ out_val) {
+// Levels >= 1: 256 summaries per workgroup, merged by the same tree.
+__global__ void __launch_bounds__(256) k_combine(const RunSummary* __restrict__ in, const uint32_t* __restrict__ m_dev,
+                                                 int k0, int level, JacRec* __restrict__ buckets,
+                                                 RunSummary* __restrict__ out, int last_level) {
+  __shared__ RunSummary sh[256];
+  int64_t count = (((int64_t)*m_dev + k0 - 1) / k0 + 255) / 256;  // summaries of level 1
+  for (int l = 1; l < level; ++l) count = (count + 255) / 256;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t m = level_size(*m_dev, level, k0, k1);
-  const int64_t s = t * k1;
-  if (s >= m) return;
-  const int64_t e = min(s + (int64_t)k1, m);
-  const Fq f;
-  const uint32_t first_key = in_key[s] & kKeyMask, last_key = in_key[e - 1] & kKeyMask;
-  const bool head_partial = s > 0 && (in_key[s - 1] & kKeyMask) == first_key;
-  const bool tail_partial = e < m && (in_key[e] & kKeyMask) == last_key;
-  out_key[2 * t] = first_key | kInvalid;
-  out_key[2 * t + 1] = last_key | kInvalid;
-  Jac acc = jac_identity(f);
-  bool has = false;
-  uint32_t cur = first_key;
-  bool first_seg = true;
-  for (int64_t x = s; x <= e; ++x) {
-    const uint32_t raw = x < e ? in_key[x] : 0xffffffffu;
-    const uint32_t key = x < e ? (raw & kKeyMask) : 0xffffffffu;
-    if (key != cur) {
-      const bool is_last = x == e;
-      if (first_seg && head_partial) {
-        out_key[2 * t] = cur | (has ? 0u : kInvalid);
-        if (has) store_jac_rec(out_val + 2 * t, acc);
-      } else if (is_last && tail_partial) {
-        out_key[2 * t + 1] = cur | (has ? 0u : kInvalid);
-        if (has) store_jac_rec(out_val + 2 * t + 1, acc);
-      } else if (has) {
-        store_jac_rec(buckets + cur, acc);
-      }
-      if (is_last) break;
-      first_seg = false;
-      cur = key;
-      acc = jac_identity(f);
-      has = false;
-    }
-    if (!(raw & kInvalid)) {
-      acc = jac_add(f, acc, load_jac_rec(in_val + x));
-      has = true;
-    }
+  if (t < count) {
+    sh[threadIdx.x] = in[t];
+  } else {
+    sh[threadIdx.x].flags = 0;
+  }
+  merge_tree(sh, buckets);
+  if (threadIdx.x == 0) {
+    if (last_level) close_runs(&sh[0], buckets);
+    else out[blockIdx.x] = sh[0];
   }
 }
 
@@ -361,7 +402,7 @@ __global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ 
   const int seg_len = 1 << job.log_seg;
   const int64_t win = local >> job.log_segs, s = local & ((1LL << job.log_segs) - 1);
   const JacRec* b = buckets + job.key_base + (win << (job.c - 1)) + s * seg_len;
-  const Fq f;
+  const FqIlp f;
   Jac r = jac_identity(f), t = jac_identity(f);
   for (int i = seg_len - 1; i >= 0; --i) {
     r = jac_add(f, r, load_jac_rec(b + i));
@@ -371,22 +412,28 @@ __global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ 
   store_jac_rec(run + g, r);
 }
 
-// One workgroup per (job, window, j): j < log_segs gives Q_j = sum of R_s over segments s with
-// bit j set; j = log_segs gives sum of T_s. Output row = the workgroup index.
+constexpr int kRedPerThread = 8;                      // segments each thread adds serially
+constexpr int kRedChunk = 256 * kRedPerThread;          // segments per workgroup
+
+// One workgroup per (job, window, j, chunk): over segments s of the chunk, j < log_segs sums
+// R_s for s with bit j set (Q_j), j = log_segs sums T_s. 256 threads x 8 segments, then a tree.
 __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ jobs, int njobs,
                                                      const JacRec* __restrict__ tot, const JacRec* __restrict__ run,
-                                                     JacRec* __restrict__ out) {
+                                                     JacRec* __restrict__ parts) {
   __shared__ JacRec sh[256];
-  const MsmJob& job = jobs[find_job<&MsmJob::sums_base>(jobs, njobs, blockIdx.x)];
-  const int64_t local = (int64_t)blockIdx.x - job.sums_base;
+  const MsmJob& job = jobs[find_job<&MsmJob::part_base>(jobs, njobs, blockIdx.x)];
+  const int64_t local = (int64_t)blockIdx.x - job.part_base;
   const int per = job.log_segs + 1;
-  const int64_t win = local / per;
-  const int j = (int)(local % per);
+  const int64_t row = local / job.chunks;
+  const int64_t chunk = local % job.chunks;
+  const int64_t win = row / per;
+  const int j = (int)(row % per);
   const int64_t segs = 1LL << job.log_segs;
-  const Fq f;
+  const FqIlp f;
   Jac acc = jac_identity(f);
   const JacRec* src = (j == job.log_segs ? tot : run) + job.seg_base + win * segs;
-  for (int64_t s = threadIdx.x; s < segs; s += 256) {
+  const int64_t lo = chunk * kRedChunk, hi = min(segs, lo + (int64_t)kRedChunk);
+  for (int64_t s = lo + threadIdx.x; s < hi; s += 256) {
     if (j == job.log_segs || ((s >> j) & 1)) acc = jac_add(f, acc, load_jac_rec(src + s));
   }
   store_jac_rec(&sh[threadIdx.x], acc);
@@ -399,26 +446,52 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) parts[blockIdx.x] = sh[0];
 }
 
-// Horner over every bit position, one thread per job: window w contributes sum T at 2^(cw)
-// and Q_j at 2^(cw + log2 L + j).
-__global__ void k_final(const MsmJob* __restrict__ jobs, int njobs, const JacRec* __restrict__ sums) {
+// One thread per (job, window, row): the row's chunk partials summed.
+__global__ void __launch_bounds__(64) k_row_sums(const MsmJob* __restrict__ jobs, int njobs, int64_t rows,
+                                                 const JacRec* __restrict__ parts, JacRec* __restrict__ row_out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= rows) return;
+  const MsmJob& job = jobs[find_job<&MsmJob::sums_base>(jobs, njobs, g)];
+  const JacRec* p = parts + job.part_base + (g - job.sums_base) * job.chunks;
+  const FqIlp f;
+  Jac r = load_jac_rec(p);
+  for (int k = 1; k < job.chunks; ++k) r = jac_add(f, r, load_jac_rec(p + k));
+  store_jac_rec(row_out + g, r);
+}
+
+// One thread per (job, window): U = sum T_s + 2^log_seg * sum_j 2^j Q_j (Horner over j).
+__global__ void __launch_bounds__(64) k_window_sums(const MsmJob* __restrict__ jobs, int njobs, int64_t total_windows,
+                                                    const JacRec* __restrict__ rows, JacRec* __restrict__ wsum) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total_windows) return;
+  const MsmJob& job = jobs[find_job<&MsmJob::win_base>(jobs, njobs, g)];
+  const int64_t win = g - job.win_base;
+  const int per = job.log_segs + 1;
+  const JacRec* p = rows + job.sums_base + win * per;
+  const FqIlp f;
+  Jac acc = jac_identity(f);
+  for (int j = job.log_segs - 1; j >= 0; --j) {
+    acc = jac_dbl(f, acc);
+    acc = jac_add(f, acc, load_jac_rec(p + j));
+  }
+  for (int k = 0; k < job.log_seg; ++k) acc = jac_dbl(f, acc);
+  acc = jac_add(f, acc, load_jac_rec(p + job.log_segs));
+  store_jac_rec(wsum + g, acc);
+}
+
+// One thread per job: Horner over the windows, sum_w 2^(cw) U_w.
+__global__ void k_final(const MsmJob* __restrict__ jobs, int njobs, const JacRec* __restrict__ wsum) {
   const int jb = blockIdx.x * blockDim.x + threadIdx.x;
   if (jb >= njobs) return;
   const MsmJob& job = jobs[jb];
-  const Fq f;
+  const FqIlp f;
   Jac acc = jac_identity(f);
-  const int per = job.log_segs + 1;
   for (int w = job.windows - 1; w >= 0; --w) {
-    const JacRec* ws = sums + job.sums_base + (int64_t)w * per;
-    for (int e = job.c - 1; e >= 0; --e) {
-      acc = jac_dbl(f, acc);
-      const int j = e - job.log_seg;
-      if (j >= 0 && j < job.log_segs) acc = jac_add(f, acc, load_jac_rec(ws + j));
-      if (e == 0) acc = jac_add(f, acc, load_jac_rec(ws + job.log_segs));
-    }
+    for (int e = 0; e < job.c; ++e) acc = jac_dbl(f, acc);
+    acc = jac_add(f, acc, load_jac_rec(wsum + job.win_base + w));
   }
   store_jac(f, job.out, acc);
 }
@@ -444,19 +517,18 @@ int default_window(int64_t n) {
 // allocates nothing (and the launch sequence can be captured into a graph).
 struct MsmPlan {
   int njobs = 0;
-  int k0 = 16, k1 = 16;
+  int k0 = 16;
   std::vector<MsmJob> jobs;
   std::vector<int64_t> level_max;
-  int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, segments = 0, sums_rows = 0;
+  int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, segments = 0, sums_rows = 0, parts = 0, windows_total = 0;
   const AffRec* points = nullptr;
   // device buffers
   void* mem = nullptr;
   MsmJob* d_jobs = nullptr;
   uint32_t *counts = nullptr, *offsets = nullptr, *m_dev = nullptr, *skey = nullptr, *sref = nullptr, *scan_ws = nullptr;
   JacRec* buckets = nullptr;
-  uint32_t* slot_key[2] = {nullptr, nullptr};
-  JacRec* slot_val[2] = {nullptr, nullptr};
-  JacRec *tot = nullptr, *run = nullptr, *sums = nullptr;
+  RunSummary* runs[2] = {nullptr, nullptr};
+  JacRec *tot = nullptr, *run = nullptr, *parts_buf = nullptr, *rows_buf = nullptr, *wsum = nullptr;
 };
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -465,13 +537,11 @@ static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, hipStream_t stream, MsmPlan** out) {
   auto plan = new MsmPlan();
   plan->points = points;
-  plan->k0 = chunk > 0 ? chunk : 16;
-  if (plan->k0 < 2) {
+  if (chunk < 0) {
     delete plan;
-    return fail("chunk_len must be at least 2");
+    return fail("chunk_len must be positive (or 0 for automatic)");
   }
-  plan->k1 = std::max(plan->k0, 4);
-  int64_t blocks = 0, keys = 0, entries = 0, segs = 0, sums = 0;
+  int64_t blocks = 0, keys = 0, entries = 0, segs = 0, sums = 0, parts = 0, wins = 0;
   for (auto& j : jobs) {
     if (j.n < 0 || j.n > 0x7fffffff) {
       delete plan;
@@ -483,18 +553,25 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
       return fail("window must be in 1..20");
     }
     j.windows = 254 / j.c + 1;
-    j.log_seg = std::min(3, j.c - 1);
+    j.log_seg = std::min(2, j.c - 1);
     j.log_segs = (j.c - 1) - j.log_seg;
+    j.chunks = (int)(((1LL << j.log_segs) + kRedChunk - 1) / kRedChunk);
     j.block_base = blocks;
     j.key_base = keys;
     j.seg_base = segs;
     j.sums_base = sums;
+    j.part_base = parts;
+    j.win_base = wins;
     blocks += std::max<int64_t>(1, (j.n + 255) / 256);
     keys += (int64_t)j.windows << (j.c - 1);
     entries += (int64_t)j.windows * j.n;
     segs += (int64_t)j.windows << j.log_segs;
     sums += (int64_t)j.windows * (j.log_segs + 1);
+    parts += (int64_t)j.windows * (j.log_segs + 1) * j.chunks;
+    wins += j.windows;
   }
+  // Bucket entries per accumulation thread: enough threads to fill the GPU (~128K), at most 32.
+  plan->k0 = chunk > 0 ? chunk : (int)std::min<int64_t>(32, std::max<int64_t>(4, entries / 131072));
   if (keys >= 0x7fffffff || entries >= 0xffffffffLL) {
     delete plan;
     return fail("MSM batch too large for 32-bit bucket keys");
@@ -506,18 +583,15 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   plan->max_entries = entries;
   plan->segments = segs;
   plan->sums_rows = sums;
-  // Accumulation levels (upper bounds): a level with more entries than one thread feeds another.
-  plan->level_max.push_back(entries);
-  while (plan->level_max.back() > (plan->level_max.size() == 1 ? plan->k0 : plan->k1)) {
-    const int k = plan->level_max.size() == 1 ? plan->k0 : plan->k1;
-    plan->level_max.push_back(2 * ((plan->level_max.back() + k - 1) / k));
-    if (plan->level_max.size() > 64) {
-      delete plan;
-      return fail("internal: MSM level plan did not converge");
-    }
-  }
-  int64_t max_slots = 2;
-  for (size_t l = 1; l < plan->level_max.size(); ++l) max_slots = std::max(max_slots, plan->level_max[l]);
+  plan->parts = parts;
+  plan->windows_total = wins;
+  // Accumulation levels: level 0 leaves one summary per workgroup of 256 x k0 entries; each
+  // further level merges 256 summaries, until a single workgroup remains.
+  const int64_t threads0 = std::max<int64_t>(1, (entries + plan->k0 - 1) / plan->k0);
+  plan->level_max.push_back((threads0 + 255) / 256);  // workgroups of level 0
+  while (plan->level_max.back() > 1) plan->level_max.push_back((plan->level_max.back() + 255) / 256);
+  int64_t max_slots = 1;
+  for (int64_t c : plan->level_max) max_slots = std::max(max_slots, c);
   plan->max_slots = max_slots;
   // one allocation for everything
   const size_t e1 = (size_t)std::max<int64_t>(entries, 1);
@@ -535,10 +609,11 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   const size_t o_sref = take(4 * e1);
   const size_t o_scan = take(4 * scan_workspace_words(keys));
   const size_t o_buckets = take(sizeof(JacRec) * (size_t)keys);
-  const size_t o_sk0 = take(4 * (size_t)max_slots), o_sk1 = take(4 * (size_t)max_slots);
-  const size_t o_sv0 = take(sizeof(JacRec) * (size_t)max_slots), o_sv1 = take(sizeof(JacRec) * (size_t)max_slots);
+  const size_t o_r0 = take(sizeof(RunSummary) * (size_t)max_slots), o_r1 = take(sizeof(RunSummary) * (size_t)max_slots);
   const size_t o_tot = take(sizeof(JacRec) * (size_t)segs), o_run = take(sizeof(JacRec) * (size_t)segs);
-  const size_t o_sums = take(sizeof(JacRec) * (size_t)sums);
+  const size_t o_parts = take(sizeof(JacRec) * (size_t)parts);
+  const size_t o_rows = take(sizeof(JacRec) * (size_t)sums);
+  const size_t o_wsum = take(sizeof(JacRec) * (size_t)wins);
   plan->mem = dev_alloc(off, stream);
   if (!plan->mem) {
     delete plan;
@@ -553,13 +628,13 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   plan->sref = reinterpret_cast<uint32_t*>(base + o_sref);
   plan->scan_ws = reinterpret_cast<uint32_t*>(base + o_scan);
   plan->buckets = reinterpret_cast<JacRec*>(base + o_buckets);
-  plan->slot_key[0] = reinterpret_cast<uint32_t*>(base + o_sk0);
-  plan->slot_key[1] = reinterpret_cast<uint32_t*>(base + o_sk1);
-  plan->slot_val[0] = reinterpret_cast<JacRec*>(base + o_sv0);
-  plan->slot_val[1] = reinterpret_cast<JacRec*>(base + o_sv1);
+  plan->runs[0] = reinterpret_cast<RunSummary*>(base + o_r0);
+  plan->runs[1] = reinterpret_cast<RunSummary*>(base + o_r1);
   plan->tot = reinterpret_cast<JacRec*>(base + o_tot);
   plan->run = reinterpret_cast<JacRec*>(base + o_run);
-  plan->sums = reinterpret_cast<JacRec*>(base + o_sums);
+  plan->parts_buf = reinterpret_cast<JacRec*>(base + o_parts);
+  plan->rows_buf = reinterpret_cast<JacRec*>(base + o_rows);
+  plan->wsum = reinterpret_cast<JacRec*>(base + o_wsum);
   hipError_t e = hipMemcpyAsync(plan->d_jobs, plan->jobs.data(), sizeof(MsmJob) * jobs.size(), hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) {
@@ -595,32 +670,37 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
                        plan->skey, plan->sref);
     ZK_LAUNCH_CHECK("msm scatter");
   }
-  const int64_t threads0 = (plan->max_entries + plan->k0 - 1) / plan->k0;
-  if (threads0 > 0) {
+  {
     Prof prof("msm_accumulate", stream);
-    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(threads0, 256)), dim3(256), 0, stream, plan->skey, plan->sref,
-                       plan->m_dev, plan->k0, plan->points, plan->buckets, plan->slot_key[0], plan->slot_val[0]);
+    const int levels = (int)plan->level_max.size();
+    hipLaunchKernelGGL(k_accumulate, dim3((unsigned)plan->level_max[0]), dim3(256), 0, stream, plan->skey, plan->sref,
+                       plan->m_dev, plan->k0, plan->points, plan->buckets, plan->runs[0], levels == 1 ? 1 : 0);
     ZK_LAUNCH_CHECK("msm accumulate");
-  }
-  int cur = 0;
-  for (size_t l = 1; l < plan->level_max.size(); ++l) {
-    const int64_t threads = (plan->level_max[l] + plan->k1 - 1) / plan->k1;
-    hipLaunchKernelGGL(k_combine, dim3(grid_for(threads, 256)), dim3(256), 0, stream, plan->slot_key[cur], plan->slot_val[cur],
-                       plan->m_dev, (int)l, plan->k0, plan->k1, plan->buckets, plan->slot_key[cur ^ 1], plan->slot_val[cur ^ 1]);
-    ZK_LAUNCH_CHECK("msm combine");
-    cur ^= 1;
+    int cur = 0;
+    for (int l = 1; l < levels; ++l) {
+      hipLaunchKernelGGL(k_combine, dim3((unsigned)plan->level_max[(size_t)l]), dim3(256), 0, stream, plan->runs[cur],
+                         plan->m_dev, plan->k0, l, plan->buckets, plan->runs[cur ^ 1], l + 1 == levels ? 1 : 0);
+      ZK_LAUNCH_CHECK("msm combine");
+      cur ^= 1;
+    }
   }
   {
     Prof prof("msm_reduce", stream);
     hipLaunchKernelGGL(k_reduce_leaf, dim3(grid_for(plan->segments, 256)), dim3(256), 0, stream, plan->d_jobs, nj,
                        plan->segments, plan->buckets, plan->tot, plan->run);
     ZK_LAUNCH_CHECK("msm reduce leaf");
-    hipLaunchKernelGGL(k_reduce_bits, dim3((unsigned)plan->sums_rows), dim3(256), 0, stream, plan->d_jobs, nj, plan->tot,
-                       plan->run, plan->sums);
+    hipLaunchKernelGGL(k_reduce_bits, dim3((unsigned)plan->parts), dim3(256), 0, stream, plan->d_jobs, nj, plan->tot,
+                       plan->run, plan->parts_buf);
     ZK_LAUNCH_CHECK("msm reduce bits");
+    hipLaunchKernelGGL(k_row_sums, dim3(grid_for(plan->sums_rows, 64)), dim3(64), 0, stream, plan->d_jobs, nj,
+                       plan->sums_rows, plan->parts_buf, plan->rows_buf);
+    ZK_LAUNCH_CHECK("msm row sums");
+    hipLaunchKernelGGL(k_window_sums, dim3(grid_for(plan->windows_total, 64)), dim3(64), 0, stream, plan->d_jobs, nj,
+                       plan->windows_total, plan->rows_buf, plan->wsum);
+    ZK_LAUNCH_CHECK("msm window sums");
   }
   Prof prof("msm_final", stream);
-  hipLaunchKernelGGL(k_final, dim3(grid_for(nj, 64)), dim3(64), 0, stream, plan->d_jobs, nj, plan->sums);
+  hipLaunchKernelGGL(k_final, dim3(grid_for(nj, 64)), dim3(64), 0, stream, plan->d_jobs, nj, plan->wsum);
   ZK_LAUNCH_CHECK("msm final");
   return 0;
 }
