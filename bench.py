@@ -16,7 +16,7 @@ each launch (zk_prof_*). Algorithmic work per launch = (nonzero signed
 digits) x 11 Fq multiplications x 136 32-bit multiply-adds, the cost basis of
 a mixed addition on 8x32-bit limbs; the peak is gfx950's measured
 v_mad_u64_u32 rate (zk_valu_peak). cpu_baseline: the oracle's Python-int
-msm_pippenger (the reference's execution model: one core) on a 2^12-point
+msm_pippenger (the reference's execution model: one core) on a 2^18-point
 sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -84,6 +84,10 @@ def main() -> int:
     ap.add_argument("--log-ntt", type=int, default=22)
     ap.add_argument("--no-extras", action="store_true", help="only the headline MSM")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs of the served workload")
+    ap.add_argument("--serve-rate", type=float, default=300.0, help="Poisson arrival rate, jobs/s")
+    ap.add_argument("--serve-budget-gib", type=float, default=24.0, help="HBM budget of the server")
+    ap.add_argument("--plain-sample", type=int, default=200, help="jobs timed through the per-call path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -129,7 +133,9 @@ def main() -> int:
     result = _result(out)
     if not result.is_on_curve():
         raise RuntimeError("bench: MSM result is not on the curve")
-    c_used = window or _default_window(gb, n)
+    from zkgpu.autotune import lookup
+
+    c_used = window or lookup("msm_window", n) or _default_window(gb, n)
     entries = signed_digit_count(scalars_host, c_used)
     mads_per_entry = 11 * 136
     acc_avg = acc_ms / max(acc_launches, 1)
@@ -142,6 +148,8 @@ def main() -> int:
     extras: dict = {}
     if not args.no_extras:
         extras.update(_ntt_and_pipeline(args, gb, BN254_FR, workloads, log))
+        extras.update(_batched(args, gb, points, workloads, log))
+        extras.update(_served(args, gb, points, log))
     cpu = None if args.no_cpu else _cpu_baseline(points, scalars_host, gb, log)
 
     line = {
@@ -161,6 +169,7 @@ def main() -> int:
             "workload": f"plain BN254 G1 Pippenger MSM, 2^{args.log_msm} device-resident points, witness-like scalars",
             "msm_points": n,
             "window_bits_signed": c_used,
+            "autotuned": bool(lookup("msm_window", n)),
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -257,14 +266,139 @@ def _ntt_and_pipeline(args, gb, Fr, workloads, log) -> dict:
     }
 
 
+def _batched(args, gb, points, workloads, log) -> dict:
+    """
+    The batched form: 64 MSMs with sizes log-uniform in [2^10, 2^20] over prefixes of the
+    2^20-point basis plus 32 forward NTTs of sizes 2^k, k uniform in {10..20}, as one graph
+    replay, against the same work through the per-call path (one zk_msm_g1 per MSM, each read
+    back before the next, as msm_pippenger_gpu returns a point; one zk_ntt per transform).
+    """
+    from zkgpu.batch import SRS, MsmBatch, NttBatch, capture
+    from zkgpu.field256_gpu import BN254_FR
+    from zkgpu.field import FIELD_MODULUS as R
+    from zkgpu.ntt import get_ntt_root
+
+    msm_sizes = workloads.log_uniform_sizes(64, 10, 20, 0x5EED10)
+    ntt_sizes = workloads.log_uniform_pow2(32, 10, 20, 0x5EED11)
+    log(f"batched: 64 MSMs ({sum(msm_sizes)} points) + 32 NTTs ({sum(ntt_sizes)} elements)")
+    srs = SRS(points)
+    pool = gb.asarray(workloads.witness_rows_fast(1 << 20, 0x5EED12))
+    scal = [pool[: n] for n in msm_sizes]
+    ntt_in = [BN254_FR.from_limbs(workloads.uniform_rows(n, 0x5EED13 + i)) for i, n in enumerate(ntt_sizes)]
+    ntt_out = [gb.empty((n, 4)) for n in ntt_sizes]
+    outs = [gb.empty((3, 4)) for _ in msm_sizes]
+    omegas = {n: gb.limbs_of(get_ntt_root(n, R)) for n in set(ntt_sizes)}
+
+    def plain() -> None:
+        for s, n, o in zip(scal, msm_sizes, outs):
+            gb.call("zk_msm_g1", points.ptr, s.ptr, n, 0, 0, o.ptr, gb.current_stream())
+            o.get()
+        for x, y, n in zip(ntt_in, ntt_out, ntt_sizes):
+            gb.call("zk_ntt", 0, x.ptr, y.ptr, n.bit_length() - 1, 0, omegas[n], None, gb.current_stream())
+        gb.synchronize()
+
+    st = gb.Stream()
+    with gb.stream(st.handle):
+        msm = MsmBatch(srs, scal)
+        ntt = NttBatch(ntt_in, ntt_out, [False] * len(ntt_sizes))
+    st.synchronize()
+    graph = capture([msm, ntt], st)
+
+    def batched() -> None:
+        with gb.stream(st.handle):
+            graph.replay()
+            msm.out.get()  # every result back on the host, as in the per-call path
+        st.synchronize()
+
+    plain()
+    batched()
+    reps = max(2, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain()
+    plain_ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batched()
+    batch_ms = (time.perf_counter() - t0) * 1e3 / reps
+    # the batched results equal the per-call ones (as group elements)
+    from zkgpu.msm_gpu import _result
+
+    got = msm.results()
+    for k in (0, 31, 63):
+        want = _result(outs[k])
+        if (got[k].is_identity() != want.is_identity()) or (not want.is_identity() and got[k].affine() != want.affine()):
+            raise RuntimeError(f"bench: batched MSM {k} differs from the per-call result")
+    log(f"batched: per-call {plain_ms:.2f} ms, batched graph {batch_ms:.2f} ms, speedup {plain_ms / batch_ms:.2f}x")
+    return {
+        "batched": {
+            "workload": "64 G1 MSMs log-uniform in [2^10, 2^20] over one 2^20 SRS + 32 Fr NTTs 2^10..2^20",
+            "per_call_ms": round(plain_ms, 3),
+            "batched_graph_ms": round(batch_ms, 3),
+            "speedup": round(plain_ms / batch_ms, 3),
+            "windows": msm.windows[:4] + ["..."],
+        }
+    }
+
+
+def _served(args, gb, points, log) -> dict:
+    """
+    The served form: Groth16 proof jobs (quotient at size n with coset shift 7 + three G1 MSMs
+    of size n over the shared 2^20 SRS), n = 2^k with k uniform in {12..20}. Capacity with all
+    jobs queued at once; latency under Poisson arrivals at a fixed rate; the per-call path
+    (one quotient call and three MSM calls per job, in order) timed on the first jobs.
+    """
+    from zkgpu.batch import SRS
+    from zkgpu.serve import ProofServer, WitnessPools, run_plain, served_workload
+
+    srs = SRS(points)
+    pools = WitnessPools((1 << 20) + (1 << 16), seed=0x5EED05)
+    budget = int(args.serve_budget_gib * 2**30)
+    server = ProofServer(srs, hbm_budget=budget, slots=2, max_batch_jobs=64)
+    log("served: warm-up (tables for every size)")
+    server.serve([pools.job(100000 + k, k, 0.0) for k in range(12, 21)])
+    gb.synchronize()
+    log(f"served: capacity, {args.serve_jobs} jobs queued at once")
+    cap = server.serve(served_workload(pools, args.serve_jobs, None, seed=0x5EED06)).summary()
+    log(f"served: capacity {cap['jobs_per_s']} jobs/s in {cap['batches']} batches")
+    gb.synchronize()
+    log(f"served: Poisson arrivals at {args.serve_rate} jobs/s")
+    rep = server.serve(served_workload(pools, args.serve_jobs, args.serve_rate, seed=0x5EED06)).summary()
+    log(f"served: p50 {rep['p50_ms']} ms, p99 {rep['p99_ms']} ms, {rep['jobs_per_s']} jobs/s")
+    gb.synchronize()
+    sample = served_workload(pools, args.plain_sample, None, seed=0x5EED06)
+    run_plain(sample[0], srs.points)
+    gb.synchronize()
+    t0 = time.perf_counter()
+    for j in sample:
+        run_plain(j, srs.points)
+    gb.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_rate = len(sample) / plain_s
+    log(f"served: per-call path {plain_rate:.1f} jobs/s on the first {len(sample)} jobs")
+    return {
+        "served": {
+            "workload": f"{args.serve_jobs} Groth16 jobs, n = 2^k, k uniform in 12..20: quotient + 3 G1 MSMs over a 2^20 SRS",
+            "rate_jobs_s": args.serve_rate,
+            "p50_ms": rep["p50_ms"],
+            "p99_ms": rep["p99_ms"],
+            "poisson": rep,
+            "capacity": cap,
+            "per_call_jobs_s": round(plain_rate, 2),
+            "per_call_sample_jobs": len(sample),
+            "capacity_vs_per_call": round(cap["jobs_per_s"] / plain_rate, 3),
+        }
+    }
+
+
 def _cpu_baseline(points, scalars_host, gb, log) -> dict:
-    """The oracle's Python msm_pippenger on 2^12 points of the same workload, one core."""
+    """The oracle's Python msm_pippenger on 2^18 points of the same workload, one core."""
     sys.path.insert(0, str(ROOT / "oracle"))
     from zkgpu.field256_gpu import BN254_FQ, _limbs_to_ints
     from zkref.curve import Point
     from zkref.msm import msm_pippenger
 
-    sample = 1 << 12
+    sample = 1 << 18
     coords = _limbs_to_ints(BN254_FQ.to_limbs(points[:sample].reshape(-1, 4)).get())
     pts = [Point(coords[2 * i], coords[2 * i + 1]) for i in range(sample)]
     sc = _limbs_to_ints(scalars_host[:sample])

@@ -55,6 +55,9 @@ int zk_event_create(void** event);
 int zk_event_destroy(void* event);
 int zk_event_record(void* event, void* stream);
 int zk_event_elapsed_ms(void* start, void* end, float* ms);
+/* *done = 1 once every piece of work before the event's record has finished, else 0; never blocks. */
+int zk_event_query(void* event, int* done);
+int zk_event_sync(void* event);
 int zk_stream_wait_event(void* stream, void* event);
 /* HIP graphs over a created stream: everything enqueued between begin and end (e.g. plan
  * launches, which allocate nothing) is recorded once and replayed by zk_graph_launch (no
@@ -101,11 +104,30 @@ int zk_field_sub(int field_id, const uint64_t* a, int a_bcast, const uint64_t* b
 int zk_ntt(int field_id, const uint64_t* in, uint64_t* out, int log_n, int inverse,
            const uint64_t omega[4], const uint64_t* coset, void* stream);
 int zk_ntt_cache_clear(void);
+/* Radix-2 stages fused per LDS-staged pass for transforms of size 2^log_n (1..10; 0 restores
+ * the default of 8), as picked by the autotuner; plans created afterwards use it. No counterpart
+ * in the reference, whose NTT runs one kernel launch per stage (src/zkgpu/ntt.py:149-203). */
+int zk_ntt_set_fusion(int log_n, int max_stages);
+int zk_ntt_get_fusion(int log_n, int* max_stages);
 /* Groth16 quotient over Fr (demo/prover_simulation.py:142-150 with coset shift g):
  * h = coset_intt((coset_ntt(intt a) * coset_ntt(intt b) - coset_ntt(intt c)) * (g^n - 1)^-1).
  * a, b, c, h: n mont rows each (h may alias none of them). */
 int zk_groth16_quotient(const uint64_t* a, const uint64_t* b, const uint64_t* c, uint64_t* h,
                         int log_n, const uint64_t shift[4], void* stream);
+/* The quotient above for count jobs of mixed sizes as one launch sequence: every job's three
+ * inverse NTTs (with their coset post-scale) in one launch per pass index, then every forward
+ * NTT, then every final inverse NTT with the fused (A B - C) / Z load. Job j reads a[j], b[j],
+ * c[j] (2^log_n[j] mont rows each) and writes h[j]; shifts holds count plain 4-limb coset
+ * shifts. The plan holds the coset evaluations and pass scratch; launching allocates nothing.
+ * No counterpart in the reference, which runs one job's transforms one call at a time
+ * (demo/prover_simulation.py:142-150). */
+int zk_groth16_plan_create(int count, const uint64_t* const* a, const uint64_t* const* b,
+                           const uint64_t* const* c, uint64_t* const* h, const int32_t* log_n,
+                           const uint64_t* shifts, void** plan, void* stream);
+int zk_groth16_plan_launch(void* plan, void* stream);
+int zk_groth16_plan_destroy(void* plan);
+/* Device bytes zk_groth16_plan_create holds for these sizes (for admission by HBM budget). */
+int zk_groth16_workspace_bytes(int count, const int32_t* log_n, uint64_t* bytes);
 /* Several Fr NTTs of different sizes as one launch per pass index (every transform's pass p
  * in the same launch). Per job j: in[j], out[j] (device addresses, in may equal out), log_n[j]
  * in 1..28, inverse[j] (the inverse includes the n^-1 scale); roots from the generator 5, as
@@ -157,6 +179,8 @@ int zk_msm_plan_create(void* srs, int count, const uint64_t* const* scalars, con
                        const int32_t* window, int chunk_len, uint64_t* out_jac, void** plan, void* stream);
 int zk_msm_plan_launch(void* plan, void* stream);
 int zk_msm_plan_destroy(void* plan);
+/* Device bytes zk_msm_plan_create holds for these sizes and windows (for admission by HBM budget). */
+int zk_msm_workspace_bytes(int count, const int64_t* n, const int32_t* window, int chunk_len, uint64_t* bytes);
 /* Plan, launch and release in one call. */
 int zk_msm_g1_batch(void* srs, int count, const uint64_t* const* scalars, const int64_t* n,
                     const int32_t* window, uint64_t* out_jac, void* stream);

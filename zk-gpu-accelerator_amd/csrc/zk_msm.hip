@@ -8,20 +8,24 @@
 //              d != 0 sends +-P_i to bucket (window, |d|). Signed digits halve the
 //              buckets per window against the reference's unsigned ones
 //              (src/zkgpu/msm_gpu.py:89-105).
-//  3. sort     counting sort of the (window, bucket) keys: counts, an exclusive
-//              scan and a scatter. Witness-like scalars put a large share of all
-//              entries into the few smallest digits (booleans make bucket 1 of
-//              window 0 hold ~15% of them), so those keys are counted and ranked
-//              in LDS first and reach global atomics once per workgroup.
-//  4. buckets  every thread takes exactly K consecutive sorted entries and adds
-//              them with mixed Jacobian + affine additions, writing each bucket
-//              that lies wholly inside its range. A bucket cut by a range
-//              boundary leaves partial sums, at most two per thread, in a slot
-//              array that stays sorted by key; the same segmented pass over the
-//              slots (full Jacobian additions) repeats until no bucket is split.
-//              The load is balanced whatever the bucket sizes, with no host sync
-//              (the reference splits buckets into chunks level by level and syncs
-//              the host per level, msm_gpu.py:287-333).
+//  3. sort     counting sort of the (window, bucket) keys with no global atomics:
+//              digits are written window-major, one workgroup per (window, range
+//              of 16384 buckets, tile of scalars) counts its tile's digits into an
+//              LDS histogram, an exclusive scan over the (bucket, tile) counts gives
+//              every tile its first position inside each bucket, and the same
+//              workgroups then place (key, point index | sign) pairs from LDS
+//              cursors. Witness-like scalars put a large share of all entries into
+//              a few buckets (booleans send ~15% of them to bucket 1 of window 0);
+//              those hot keys only ever meet LDS atomics.
+//  4. buckets  a persistent grid splits the m sorted entries evenly, ceil(m / threads)
+//              consecutive entries per thread, added with mixed Jacobian + affine
+//              additions; a run of equal keys strictly inside a thread's range is a
+//              whole bucket and is written at once. The runs cut by range
+//              boundaries are merged by a tree of run summaries (workgroup trees in
+//              LDS, then levels of 256), each merge at most one addition. The load
+//              is balanced whatever the bucket sizes, with no host sync (the
+//              reference splits buckets into chunks level by level and syncs the
+//              host per level, msm_gpu.py:287-333).
 //  5. reduce   per window, sum_v v * B_v: threads take segments of L buckets with
 //              running sums (T = sum (i+1) B, R = sum B); a window's total is
 //              sum_s T_s + L * sum_j 2^j Q_j with Q_j = sum of R_s over segments s
@@ -44,7 +48,6 @@
 namespace zk {
 
 using Fq = FqConst;
-constexpr int kHot = 32;                      // digits 1..kHot are aggregated in LDS
 
 // ---- packed records -------------------------------------------------------
 
@@ -182,8 +185,12 @@ struct MsmJob {
   int64_t sums_base;        // first (window, bit) row of the bucket reduction
   int64_t part_base;        // first (window, bit, chunk) partial of the bucket reduction
   int64_t win_base;         // first per-window sum
+  int64_t dig_base;         // first signed digit (window-major: window w at dig_base + w * n)
+  int64_t sort_base;        // first sort block (window, bucket range, scalar tile)
+  int64_t cnt_base;         // first (bucket, tile) count
+  int64_t tile;             // scalars per sort tile
   uint64_t* out;            // 12 limbs, the Jacobian result
-  int c, windows, log_seg, log_segs, chunks;
+  int c, windows, log_seg, log_segs, chunks, tiles, ranges;
 };
 
 // Largest j with base(j) <= x for a field of the job table (bases increase).
@@ -198,46 +205,88 @@ __device__ __forceinline__ int find_job(const MsmJob* jobs, int njobs, int64_t x
   return lo;
 }
 
-// mode 0: count keys; mode 1: scatter (key, ref) into sorted position via cursors.
-// Each block handles 256 scalars of one job (jobs are padded to whole blocks).
-template <int mode>
-__global__ void __launch_bounds__(256) k_digits(const MsmJob* __restrict__ jobs, int njobs,
-                                                uint32_t* __restrict__ counts_or_cursor, uint32_t* __restrict__ skey,
-                                                uint32_t* __restrict__ sref) {
-  __shared__ uint32_t hot_cnt[kHot];
-  __shared__ uint32_t hot_base[kHot];
-  const int jb = find_job<&MsmJob::block_base>(jobs, njobs, blockIdx.x);
-  const MsmJob job = jobs[jb];
+// Signed digits of every scalar: digits[dig_base + w * n + i] = d (0 for none).
+__global__ void __launch_bounds__(256) k_signed_digits(const MsmJob* __restrict__ jobs, int njobs,
+                                                       int32_t* __restrict__ digits) {
+  const MsmJob& job = jobs[find_job<&MsmJob::block_base>(jobs, njobs, blockIdx.x)];
   const int64_t i = ((int64_t)blockIdx.x - job.block_base) * 256 + threadIdx.x;
-  const bool active = i < job.n;
-  uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (active) load_scalar(job.scalars + 4 * i, w);
-  const int c = job.c;
-  const uint32_t buckets = 1u << (c - 1);
-  const int hot = (int)min((uint32_t)kHot, buckets);
+  if (i >= job.n) return;
+  uint32_t w[8];
+  load_scalar(job.scalars + 4 * i, w);
   uint32_t carry = 0;
-  for (int win = 0; win < job.windows; ++win) {
-    if (threadIdx.x < kHot) hot_cnt[threadIdx.x] = 0;
-    __syncthreads();
-    int d = active ? signed_digit(w, win, c, carry) : 0;
-    uint32_t mag = (uint32_t)(d < 0 ? -d : d);
-    uint32_t key = (uint32_t)job.key_base + (uint32_t)win * buckets + (mag - 1);
-    bool is_hot = mag != 0 && (int)mag <= hot;
-    uint32_t local = 0;
-    if (is_hot) local = atomicAdd(&hot_cnt[mag - 1], 1u);
-    if (mode == 0 && mag != 0 && !is_hot) atomicAdd(&counts_or_cursor[key], 1u);
-    __syncthreads();
-    if (threadIdx.x < hot && hot_cnt[threadIdx.x]) {
-      uint32_t k = (uint32_t)job.key_base + (uint32_t)win * buckets + threadIdx.x;
-      hot_base[threadIdx.x] = atomicAdd(&counts_or_cursor[k], hot_cnt[threadIdx.x]);
+  for (int win = 0; win < job.windows; ++win) digits[job.dig_base + (int64_t)win * job.n + i] = signed_digit(w, win, job.c, carry);
+}
+
+constexpr int kSortRange = 16384;  // buckets per sort block (its LDS holds one counter per bucket)
+
+// Sort block geometry: (window, bucket range, scalar tile) of one job.
+struct SortBlock {
+  const MsmJob* job;
+  int win, range, tile;
+};
+
+__device__ __forceinline__ SortBlock sort_block(const MsmJob* __restrict__ jobs, int njobs) {
+  const MsmJob* job = &jobs[find_job<&MsmJob::sort_base>(jobs, njobs, blockIdx.x)];
+  const int64_t local = (int64_t)blockIdx.x - job->sort_base;
+  SortBlock b;
+  b.job = job;
+  b.tile = (int)(local % job->tiles);
+  const int64_t rest = local / job->tiles;
+  b.range = (int)(rest % job->ranges);
+  b.win = (int)(rest / job->ranges);
+  return b;
+}
+
+// Counts, per (bucket, tile), the digits of one tile that fall into this block's bucket range;
+// counts are laid out bucket-major, tile-minor, so their exclusive scan gives each
+// (bucket, tile) its first sorted position.
+__global__ void __launch_bounds__(256) k_sort_count(const MsmJob* __restrict__ jobs, int njobs,
+                                                    const int32_t* __restrict__ digits, uint32_t* __restrict__ counts) {
+  extern __shared__ uint32_t hist[];
+  const SortBlock sb = sort_block(jobs, njobs);
+  const MsmJob& job = *sb.job;
+  const int64_t buckets = 1LL << (job.c - 1);
+  const int64_t r0 = (int64_t)sb.range * kSortRange;
+  const int range = (int)min((int64_t)kSortRange, buckets - r0);
+  for (int k = threadIdx.x; k < range; k += 256) hist[k] = 0;
+  __syncthreads();
+  const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.n, lo + job.tile);
+  const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.n;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
+    const int32_t v = d[i];
+    const int64_t b = (int64_t)(v < 0 ? -v : v) - 1 - r0;  // bucket index within the range
+    if (v != 0 && b >= 0 && b < range) atomicAdd(&hist[b], 1u);
+  }
+  __syncthreads();
+  uint32_t* out = counts + job.cnt_base + ((int64_t)sb.win * buckets + r0) * job.tiles + sb.tile;
+  for (int k = threadIdx.x; k < range; k += 256) out[(int64_t)k * job.tiles] = hist[k];
+}
+
+// Places each digit's (key, point ref) at its bucket's next free position: the block's LDS
+// holds the running positions of its (bucket, tile) slots, so no global atomics are needed.
+__global__ void __launch_bounds__(256) k_sort_place(const MsmJob* __restrict__ jobs, int njobs,
+                                                    const int32_t* __restrict__ digits, const uint32_t* __restrict__ offsets,
+                                                    uint32_t* __restrict__ skey, uint32_t* __restrict__ sref) {
+  extern __shared__ uint32_t cur[];
+  const SortBlock sb = sort_block(jobs, njobs);
+  const MsmJob& job = *sb.job;
+  const int64_t buckets = 1LL << (job.c - 1);
+  const int64_t r0 = (int64_t)sb.range * kSortRange;
+  const int range = (int)min((int64_t)kSortRange, buckets - r0);
+  const uint32_t* off = offsets + job.cnt_base + ((int64_t)sb.win * buckets + r0) * job.tiles + sb.tile;
+  for (int k = threadIdx.x; k < range; k += 256) cur[k] = off[(int64_t)k * job.tiles];
+  __syncthreads();
+  const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.n, lo + job.tile);
+  const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.n;
+  const uint32_t key0 = (uint32_t)(job.key_base + (int64_t)sb.win * buckets + r0);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
+    const int32_t v = d[i];
+    const int64_t b = (int64_t)(v < 0 ? -v : v) - 1 - r0;
+    if (v != 0 && b >= 0 && b < range) {
+      const uint32_t pos = atomicAdd(&cur[b], 1u);
+      skey[pos] = key0 + (uint32_t)b;
+      sref[pos] = (uint32_t)i | (v < 0 ? 0x80000000u : 0u);
     }
-    __syncthreads();
-    if (mode == 1 && mag != 0) {
-      uint32_t pos = is_hot ? hot_base[mag - 1] + local : atomicAdd(&counts_or_cursor[key], 1u);
-      skey[pos] = key;
-      sref[pos] = (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
-    }
-    __syncthreads();
   }
 }
 
@@ -300,9 +349,10 @@ __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, J
   l->flags = kValid;
 }
 
-// Tree over the 256 summaries of a workgroup in LDS; the result is left in sh[0].
+// Tree over the N summaries of a workgroup in LDS; the result is left in sh[0].
+template <int N>
 __device__ __forceinline__ void merge_tree(RunSummary* sh, JacRec* __restrict__ buckets) {
-  for (int d = 1; d < 256; d <<= 1) {
+  for (int d = 1; d < N; d <<= 1) {
     __syncthreads();
     if ((threadIdx.x & (2 * d - 1)) == 0) merge_runs(&sh[threadIdx.x], &sh[threadIdx.x + d], buckets);
   }
@@ -316,21 +366,26 @@ __device__ __forceinline__ void close_runs(const RunSummary* sm, JacRec* __restr
   if (!(sm->flags & kSingle)) buckets[sm->lk] = sm->lsum;
 }
 
-// Level 0: K consecutive sorted entries per thread (mixed additions), then a workgroup tree.
-__global__ void __launch_bounds__(256) k_accumulate(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sref,
-                                                    const uint32_t* __restrict__ m_dev, int k0,
-                                                    const AffRec* __restrict__ pts, JacRec* __restrict__ buckets,
-                                                    RunSummary* __restrict__ out, int last_level) {
-  __shared__ RunSummary sh[256];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+constexpr int kAccThreads = 64;      // one wave per accumulation workgroup: its tree needs 15 KB of LDS
+constexpr int kCombineThreads = 256;
+
+// Level 0, a persistent grid: the m sorted entries are split evenly, ceil(m / threads) consecutive
+// entries per thread (mixed additions), and each workgroup merges its threads' run summaries.
+__global__ void __launch_bounds__(kAccThreads)
+k_accumulate(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sref, const uint32_t* __restrict__ m_dev,
+             const AffRec* __restrict__ pts, JacRec* __restrict__ buckets, RunSummary* __restrict__ out, int last_level) {
+  __shared__ RunSummary sh[kAccThreads];
+  const int64_t threads = (int64_t)gridDim.x * kAccThreads;
+  const int64_t t = (int64_t)blockIdx.x * kAccThreads + threadIdx.x;
   const int64_t m = *m_dev;
-  const int64_t s = t * k0;
+  const int64_t k = (m + threads - 1) / threads;
+  const int64_t s = t * k;
   const Fq f;
   RunSummary me;
   me.flags = 0;
   me.fk = me.lk = 0;
   if (s < m) {
-    const int64_t e = min(s + (int64_t)k0, m);
+    const int64_t e = min(s + k, m);
     Jac acc = jac_identity(f);
     uint32_t cur = skey[s];
     bool first = true;
@@ -363,27 +418,25 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t* __restrict__
     }
   }
   sh[threadIdx.x] = me;
-  merge_tree(sh, buckets);
+  merge_tree<kAccThreads>(sh, buckets);
   if (threadIdx.x == 0) {
     if (last_level) close_runs(&sh[0], buckets);
     else out[blockIdx.x] = sh[0];
   }
 }
 
-// Levels >= 1: 256 summaries per workgroup, merged by the same tree.
-__global__ void __launch_bounds__(256) k_combine(const RunSummary* __restrict__ in, const uint32_t* __restrict__ m_dev,
-                                                 int k0, int level, JacRec* __restrict__ buckets,
-                                                 RunSummary* __restrict__ out, int last_level) {
-  __shared__ RunSummary sh[256];
-  int64_t count = (((int64_t)*m_dev + k0 - 1) / k0 + 255) / 256;  // summaries of level 1
-  for (int l = 1; l < level; ++l) count = (count + 255) / 256;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Levels >= 1: the `count` summaries of the previous level, 256 per workgroup, merged by the same tree.
+__global__ void __launch_bounds__(kCombineThreads) k_combine(const RunSummary* __restrict__ in, int64_t count,
+                                                             JacRec* __restrict__ buckets, RunSummary* __restrict__ out,
+                                                             int last_level) {
+  __shared__ RunSummary sh[kCombineThreads];
+  const int64_t t = (int64_t)blockIdx.x * kCombineThreads + threadIdx.x;
   if (t < count) {
     sh[threadIdx.x] = in[t];
   } else {
     sh[threadIdx.x].flags = 0;
   }
-  merge_tree(sh, buckets);
+  merge_tree<kCombineThreads>(sh, buckets);
   if (threadIdx.x == 0) {
     if (last_level) close_runs(&sh[0], buckets);
     else out[blockIdx.x] = sh[0];
@@ -517,10 +570,12 @@ int default_window(int64_t n) {
 // allocates nothing (and the launch sequence can be captured into a graph).
 struct MsmPlan {
   int njobs = 0;
-  int k0 = 16;
   std::vector<MsmJob> jobs;
   std::vector<int64_t> level_max;
   int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, segments = 0, sums_rows = 0, parts = 0, windows_total = 0;
+  int64_t sort_blocks = 0, ncounts = 0;
+  size_t bytes = 0;
+  int32_t* digits = nullptr;
   const AffRec* points = nullptr;
   // device buffers
   void* mem = nullptr;
@@ -533,25 +588,20 @@ struct MsmPlan {
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// jobs: scalars, n, c (0 = default) and out filled in by the caller; the rest is planned here.
-int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, hipStream_t stream, MsmPlan** out) {
-  auto plan = new MsmPlan();
-  plan->points = points;
-  if (chunk < 0) {
-    delete plan;
-    return fail("chunk_len must be positive (or 0 for automatic)");
-  }
-  int64_t blocks = 0, keys = 0, entries = 0, segs = 0, sums = 0, parts = 0, wins = 0;
+// Byte offsets of every buffer inside a plan's single device allocation.
+struct MsmLayout {
+  size_t jobs, counts, offsets, digits, m, skey, sref, scan, buckets, r0, r1, tot, run, parts, rows, wsum, total;
+};
+
+// jobs: scalars, n, c (0 = default) and out filled in by the caller; the rest is planned here,
+// without touching the device.
+static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk, MsmLayout* L) {
+  if (chunk < 0) return fail("chunk_len must be positive (or 0 for automatic)");
+  int64_t blocks = 0, keys = 0, entries = 0, segs = 0, sums = 0, parts = 0, wins = 0, sort_blocks = 0, cnts = 0;
   for (auto& j : jobs) {
-    if (j.n < 0 || j.n > 0x7fffffff) {
-      delete plan;
-      return fail("MSM size must be in [0, 2^31)");
-    }
+    if (j.n < 0 || j.n > 0x7fffffff) return fail("MSM size must be in [0, 2^31)");
     if (j.c <= 0) j.c = default_window(j.n);
-    if (j.c > 20) {
-      delete plan;
-      return fail("window must be in 1..20");
-    }
+    if (j.c > 20) return fail("window must be in 1..20");
     j.windows = 254 / j.c + 1;
     j.log_seg = std::min(2, j.c - 1);
     j.log_segs = (j.c - 1) - j.log_seg;
@@ -562,6 +612,17 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
     j.sums_base = sums;
     j.part_base = parts;
     j.win_base = wins;
+    // sort geometry: windows x bucket ranges x scalar tiles, about a thousand blocks for large jobs
+    const int64_t nb = 1LL << (j.c - 1);
+    j.ranges = (int)((nb + kSortRange - 1) / kSortRange);
+    const int64_t want = (1024 + (int64_t)j.windows * j.ranges - 1) / ((int64_t)j.windows * j.ranges);
+    j.tiles = (int)std::max<int64_t>(1, std::min<int64_t>(want, (j.n + 2047) / 2048));
+    j.tile = std::max<int64_t>(1, (j.n + j.tiles - 1) / j.tiles);
+    j.dig_base = entries;
+    j.sort_base = sort_blocks;
+    j.cnt_base = cnts;
+    sort_blocks += (int64_t)j.windows * j.ranges * j.tiles;
+    cnts += (int64_t)j.windows * nb * j.tiles;
     blocks += std::max<int64_t>(1, (j.n + 255) / 256);
     keys += (int64_t)j.windows << (j.c - 1);
     entries += (int64_t)j.windows * j.n;
@@ -570,12 +631,22 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
     parts += (int64_t)j.windows * (j.log_segs + 1) * j.chunks;
     wins += j.windows;
   }
-  // Bucket entries per accumulation thread: enough threads to fill the GPU (~128K), at most 32.
-  plan->k0 = chunk > 0 ? chunk : (int)std::min<int64_t>(32, std::max<int64_t>(4, entries / 131072));
-  if (keys >= 0x7fffffff || entries >= 0xffffffffLL) {
-    delete plan;
-    return fail("MSM batch too large for 32-bit bucket keys");
+  if (keys >= 0x7fffffff || entries >= 0xffffffffLL) return fail("MSM batch too large for 32-bit bucket keys");
+  // Accumulation grid: every resident workgroup slot of the GPU once (the kernel splits the sorted
+  // entries evenly over its threads), or, with chunk_len > 0, enough workgroups for chunk_len
+  // entries per thread at the largest possible entry count.
+  int64_t grid0;
+  if (chunk > 0) {
+    grid0 = (entries + (int64_t)chunk * kAccThreads - 1) / ((int64_t)chunk * kAccThreads);
+  } else {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_accumulate, kAccThreads, 0) != hipSuccess || per_cu < 1) {
+      (void)hipGetLastError();
+      per_cu = 1;
+    }
+    grid0 = std::min<int64_t>((int64_t)device_cus() * per_cu, (entries + 4 * kAccThreads - 1) / (4 * kAccThreads));
   }
+  grid0 = std::max<int64_t>(1, grid0);
   plan->jobs = jobs;
   plan->njobs = (int)jobs.size();
   plan->blocks = blocks;
@@ -583,17 +654,17 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   plan->max_entries = entries;
   plan->segments = segs;
   plan->sums_rows = sums;
+  plan->sort_blocks = sort_blocks;
+  plan->ncounts = cnts;
   plan->parts = parts;
   plan->windows_total = wins;
-  // Accumulation levels: level 0 leaves one summary per workgroup of 256 x k0 entries; each
-  // further level merges 256 summaries, until a single workgroup remains.
-  const int64_t threads0 = std::max<int64_t>(1, (entries + plan->k0 - 1) / plan->k0);
-  plan->level_max.push_back((threads0 + 255) / 256);  // workgroups of level 0
-  while (plan->level_max.back() > 1) plan->level_max.push_back((plan->level_max.back() + 255) / 256);
+  // Accumulation levels: level 0 leaves one run summary per workgroup; each further level merges
+  // 256 summaries per workgroup, until a single workgroup remains.
+  plan->level_max.assign(1, grid0);
+  while (plan->level_max.back() > 1) plan->level_max.push_back((plan->level_max.back() + kCombineThreads - 1) / kCombineThreads);
   int64_t max_slots = 1;
   for (int64_t c : plan->level_max) max_slots = std::max(max_slots, c);
   plan->max_slots = max_slots;
-  // one allocation for everything
   const size_t e1 = (size_t)std::max<int64_t>(entries, 1);
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -601,46 +672,66 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
     off += align_up(bytes);
     return at;
   };
-  const size_t o_jobs = take(sizeof(MsmJob) * jobs.size());
-  const size_t o_counts = take(4 * (size_t)(keys + 1));
-  const size_t o_offsets = take(4 * (size_t)(keys + 1));
-  const size_t o_m = take(16);
-  const size_t o_skey = take(4 * e1);
-  const size_t o_sref = take(4 * e1);
-  const size_t o_scan = take(4 * scan_workspace_words(keys));
-  const size_t o_buckets = take(sizeof(JacRec) * (size_t)keys);
-  const size_t o_r0 = take(sizeof(RunSummary) * (size_t)max_slots), o_r1 = take(sizeof(RunSummary) * (size_t)max_slots);
-  const size_t o_tot = take(sizeof(JacRec) * (size_t)segs), o_run = take(sizeof(JacRec) * (size_t)segs);
-  const size_t o_parts = take(sizeof(JacRec) * (size_t)parts);
-  const size_t o_rows = take(sizeof(JacRec) * (size_t)sums);
-  const size_t o_wsum = take(sizeof(JacRec) * (size_t)wins);
-  plan->mem = dev_alloc(off, stream);
+  L->jobs = take(sizeof(MsmJob) * jobs.size());
+  L->counts = take(4 * (size_t)(cnts + 1));
+  L->offsets = take(4 * (size_t)(cnts + 1));
+  L->digits = take(4 * e1);
+  L->m = take(16);
+  L->skey = take(4 * e1);
+  L->sref = take(4 * e1);
+  L->scan = take(4 * scan_workspace_words(cnts));
+  L->buckets = take(sizeof(JacRec) * (size_t)keys);
+  L->r0 = take(sizeof(RunSummary) * (size_t)max_slots);
+  L->r1 = take(sizeof(RunSummary) * (size_t)max_slots);
+  L->tot = take(sizeof(JacRec) * (size_t)segs);
+  L->run = take(sizeof(JacRec) * (size_t)segs);
+  L->parts = take(sizeof(JacRec) * (size_t)parts);
+  L->rows = take(sizeof(JacRec) * (size_t)sums);
+  L->wsum = take(sizeof(JacRec) * (size_t)wins);
+  L->total = off;
+  return 0;
+}
+
+int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, hipStream_t stream, MsmPlan** out) {
+  auto plan = new MsmPlan();
+  plan->points = points;
+  MsmLayout L;
+  if (int rc = msm_plan_geometry(plan, jobs, chunk, &L)) {
+    delete plan;
+    return rc;
+  }
+  plan->mem = dev_alloc(L.total, stream);
   if (!plan->mem) {
     delete plan;
     return 1;
   }
+  plan->bytes = L.total;
   char* base = static_cast<char*>(plan->mem);
-  plan->d_jobs = reinterpret_cast<MsmJob*>(base + o_jobs);
-  plan->counts = reinterpret_cast<uint32_t*>(base + o_counts);
-  plan->offsets = reinterpret_cast<uint32_t*>(base + o_offsets);
-  plan->m_dev = reinterpret_cast<uint32_t*>(base + o_m);
-  plan->skey = reinterpret_cast<uint32_t*>(base + o_skey);
-  plan->sref = reinterpret_cast<uint32_t*>(base + o_sref);
-  plan->scan_ws = reinterpret_cast<uint32_t*>(base + o_scan);
-  plan->buckets = reinterpret_cast<JacRec*>(base + o_buckets);
-  plan->runs[0] = reinterpret_cast<RunSummary*>(base + o_r0);
-  plan->runs[1] = reinterpret_cast<RunSummary*>(base + o_r1);
-  plan->tot = reinterpret_cast<JacRec*>(base + o_tot);
-  plan->run = reinterpret_cast<JacRec*>(base + o_run);
-  plan->parts_buf = reinterpret_cast<JacRec*>(base + o_parts);
-  plan->rows_buf = reinterpret_cast<JacRec*>(base + o_rows);
-  plan->wsum = reinterpret_cast<JacRec*>(base + o_wsum);
-  hipError_t e = hipMemcpyAsync(plan->d_jobs, plan->jobs.data(), sizeof(MsmJob) * jobs.size(), hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  if (e != hipSuccess) {
-    dev_free(plan->mem);
-    delete plan;
-    return fail_hip(e, "MSM plan upload");
+  plan->d_jobs = reinterpret_cast<MsmJob*>(base + L.jobs);
+  plan->counts = reinterpret_cast<uint32_t*>(base + L.counts);
+  plan->offsets = reinterpret_cast<uint32_t*>(base + L.offsets);
+  plan->digits = reinterpret_cast<int32_t*>(base + L.digits);
+  plan->m_dev = reinterpret_cast<uint32_t*>(base + L.m);
+  plan->skey = reinterpret_cast<uint32_t*>(base + L.skey);
+  plan->sref = reinterpret_cast<uint32_t*>(base + L.sref);
+  plan->scan_ws = reinterpret_cast<uint32_t*>(base + L.scan);
+  plan->buckets = reinterpret_cast<JacRec*>(base + L.buckets);
+  plan->runs[0] = reinterpret_cast<RunSummary*>(base + L.r0);
+  plan->runs[1] = reinterpret_cast<RunSummary*>(base + L.r1);
+  plan->tot = reinterpret_cast<JacRec*>(base + L.tot);
+  plan->run = reinterpret_cast<JacRec*>(base + L.run);
+  plan->parts_buf = reinterpret_cast<JacRec*>(base + L.parts);
+  plan->rows_buf = reinterpret_cast<JacRec*>(base + L.rows);
+  plan->wsum = reinterpret_cast<JacRec*>(base + L.wsum);
+  // Stream-ordered upload from the plan's own job table, which lives as long as the plan.
+  if (!plan->jobs.empty()) {
+    hipError_t e = hipMemcpyAsync(plan->d_jobs, plan->jobs.data(), sizeof(MsmJob) * plan->jobs.size(),
+                                  hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) {
+      dev_free(plan->mem);
+      delete plan;
+      return fail_hip(e, "MSM plan upload");
+    }
   }
   *out = plan;
   return 0;
@@ -656,30 +747,32 @@ void msm_plan_destroy(MsmPlan* plan) {
 int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
   if (plan->njobs == 0) return 0;
   const int nj = plan->njobs;
-  hipLaunchKernelGGL(k_fill_zero<uint32_t>, dim3(grid_for(plan->nkeys, 256)), dim3(256), 0, stream, plan->counts, plan->nkeys);
   hipLaunchKernelGGL(k_fill_zero<uint32_t>, dim3(grid_for(plan->nkeys * 28, 256)), dim3(256), 0, stream,
                      reinterpret_cast<uint32_t*>(plan->buckets), plan->nkeys * 28);
   ZK_LAUNCH_CHECK("msm clear");
-  hipLaunchKernelGGL(k_digits<0>, dim3((unsigned)plan->blocks), dim3(256), 0, stream, plan->d_jobs, nj, plan->counts,
-                     (uint32_t*)nullptr, (uint32_t*)nullptr);
-  ZK_LAUNCH_CHECK("msm count");
-  if (int rc = scan_u32_ws(plan->counts, plan->offsets, plan->nkeys, plan->m_dev, plan->scan_ws, stream)) return rc;
   {
-    Prof prof("msm_scatter", stream);
-    hipLaunchKernelGGL(k_digits<1>, dim3((unsigned)plan->blocks), dim3(256), 0, stream, plan->d_jobs, nj, plan->offsets,
-                       plan->skey, plan->sref);
-    ZK_LAUNCH_CHECK("msm scatter");
+    Prof prof("msm_sort", stream);
+    hipLaunchKernelGGL(k_signed_digits, dim3((unsigned)plan->blocks), dim3(256), 0, stream, plan->d_jobs, nj, plan->digits);
+    ZK_LAUNCH_CHECK("msm digits");
+    hipLaunchKernelGGL(k_sort_count, dim3((unsigned)plan->sort_blocks), dim3(256), kSortRange * 4, stream, plan->d_jobs, nj,
+                       plan->digits, plan->counts);
+    ZK_LAUNCH_CHECK("msm sort count");
+    if (int rc = scan_u32_ws(plan->counts, plan->offsets, plan->ncounts, plan->m_dev, plan->scan_ws, stream)) return rc;
+    hipLaunchKernelGGL(k_sort_place, dim3((unsigned)plan->sort_blocks), dim3(256), kSortRange * 4, stream, plan->d_jobs, nj,
+                       plan->digits, plan->offsets, plan->skey, plan->sref);
+    ZK_LAUNCH_CHECK("msm sort place");
   }
   {
     Prof prof("msm_accumulate", stream);
     const int levels = (int)plan->level_max.size();
-    hipLaunchKernelGGL(k_accumulate, dim3((unsigned)plan->level_max[0]), dim3(256), 0, stream, plan->skey, plan->sref,
-                       plan->m_dev, plan->k0, plan->points, plan->buckets, plan->runs[0], levels == 1 ? 1 : 0);
+    hipLaunchKernelGGL(k_accumulate, dim3((unsigned)plan->level_max[0]), dim3(kAccThreads), 0, stream, plan->skey,
+                       plan->sref, plan->m_dev, plan->points, plan->buckets, plan->runs[0], levels == 1 ? 1 : 0);
     ZK_LAUNCH_CHECK("msm accumulate");
     int cur = 0;
     for (int l = 1; l < levels; ++l) {
-      hipLaunchKernelGGL(k_combine, dim3((unsigned)plan->level_max[(size_t)l]), dim3(256), 0, stream, plan->runs[cur],
-                         plan->m_dev, plan->k0, l, plan->buckets, plan->runs[cur ^ 1], l + 1 == levels ? 1 : 0);
+      hipLaunchKernelGGL(k_combine, dim3((unsigned)plan->level_max[(size_t)l]), dim3(kCombineThreads), 0, stream,
+                         plan->runs[cur], plan->level_max[(size_t)l - 1], plan->buckets, plan->runs[cur ^ 1],
+                         l + 1 == levels ? 1 : 0);
       ZK_LAUNCH_CHECK("msm combine");
       cur ^= 1;
     }
@@ -792,6 +885,20 @@ int zk_msm_plan_create(void* srs, int count, const uint64_t* const* scalars, con
   MsmPlan* p = nullptr;
   if (int rc = msm_plan_create(s->pts, jobs, chunk_len, (hipStream_t)stream, &p)) return rc;
   *plan = p;
+  return 0;
+}
+
+int zk_msm_workspace_bytes(int count, const int64_t* n, const int32_t* window, int chunk_len, uint64_t* bytes) {
+  std::vector<MsmJob> jobs((size_t)std::max(count, 0));
+  for (int j = 0; j < count; ++j) {
+    jobs[(size_t)j] = MsmJob{};
+    jobs[(size_t)j].n = n[j];
+    jobs[(size_t)j].c = window ? window[j] : 0;
+  }
+  MsmPlan plan;
+  MsmLayout L;
+  if (int rc = msm_plan_geometry(&plan, jobs, chunk_len, &L)) return rc;
+  *bytes = L.total;
   return 0;
 }
 

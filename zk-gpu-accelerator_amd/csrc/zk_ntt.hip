@@ -37,8 +37,6 @@
 
 namespace zk {
 
-void fr_roots(int log_n, U256* omega, U256* omega_inv);  // zk_pipeline.hip
-
 constexpr int kTile = 1024;  // rows per workgroup tile
 constexpr int kThreads = 256;
 
@@ -310,13 +308,16 @@ struct PassPlan {
 };
 
 
+// Stages fused per pass, per transform size (0: the default of 8); set by zk_ntt_set_fusion.
+int g_fusion[29] = {0};
+
 std::vector<PassPlan> plan_passes(int log_n) {
   std::vector<PassPlan> out;
-  if (log_n <= 10) {
+  const int max_k = g_fusion[log_n] > 0 ? g_fusion[log_n] : 8;
+  if (log_n <= 10 && max_k >= log_n) {
     out.push_back({0, log_n, 0});
     return out;
   }
-  const int max_k = 8;
   int passes = (log_n + max_k - 1) / max_k;
   int s0 = 0;
   for (int p = 0; p < passes; ++p) {
@@ -407,112 +408,121 @@ int ntt_run(int field_id, const uint64_t* in, uint64_t* out, int log_n, const ui
 
 // ---- batched transforms ---------------------------------------------------------
 
-struct NttPlan {
-  std::vector<std::vector<NttPassDesc>> passes;  // by pass index, all transforms with that pass
-  std::vector<NttPassDesc*> d_passes;
-  std::vector<int64_t> tiles;
-  void* mem = nullptr;
-};
+int64_t transform_scratch_rows(int log_n) { return plan_passes(log_n).size() > 1 ? (1LL << log_n) : 0; }
 
-int ntt_plan_create(int count, const uint64_t* const* in, uint64_t* const* out, const int32_t* log_n,
-                    const int32_t* inverse, hipStream_t stream, NttPlan** result) {
-  auto plan = new NttPlan();
-  // scratch for every multi-pass transform plus the device descriptor tables
+int batch_plan_create(const std::vector<TransformSpec>& specs, hipStream_t stream, BatchPlan** result) {
+  int stages = 0;
+  for (const TransformSpec& t : specs) {
+    if (t.log_n < 1 || t.log_n > 28) return fail("batched transform sizes must be 2^1..2^28");
+    if (t.stage < 0) return fail("negative transform stage");
+    stages = std::max(stages, t.stage + 1);
+  }
+  // pass scratch: transforms of one stage run together, stages one after another
+  std::vector<size_t> stage_rows((size_t)stages, 0);
+  for (const TransformSpec& t : specs) stage_rows[(size_t)t.stage] += (size_t)transform_scratch_rows(t.log_n);
   size_t tmp_rows = 0;
-  for (int j = 0; j < count; ++j) {
-    if (log_n[j] < 1 || log_n[j] > 28) {
-      delete plan;
-      return fail("batched NTT sizes must be 2^1..2^28");
-    }
-    if (plan_passes(log_n[j]).size() > 1) tmp_rows += (size_t)1 << log_n[j];
-  }
-  std::vector<std::vector<NttPassDesc>> per_job;
-  for (int j = 0; j < count; ++j) {
-    U256 w, wi;
-    fr_roots(log_n[j], &w, &wi);
-    const uint64_t* tw = nullptr;
-    if (int rc = twiddles(0, log_n[j], inverse[j] ? wi.w : w.w, stream, &tw)) {
-      delete plan;
-      return rc;
-    }
-    const uint64_t* post = nullptr;
-    const uint64_t one[4] = {1, 0, 0, 0};
-    if (inverse[j]) {
-      if (int rc = scale_row(0, log_n[j], one, stream, &post)) {
-        delete plan;
-        return rc;
-      }
-    }
-    per_job.push_back(pass_descs(in[j], out[j], nullptr, log_n[j], tw, nullptr, post, false, QuotientArgs{}));
-  }
-  size_t max_passes = 0;
-  for (auto& v : per_job) max_passes = std::max(max_passes, v.size());
-  plan->passes.resize(max_passes);
+  for (size_t r : stage_rows) tmp_rows = std::max(tmp_rows, r);
+  // descriptors per (stage, pass index)
+  std::vector<std::vector<std::vector<NttPassDesc>>> launches((size_t)stages);
+  std::vector<size_t> tmp_at((size_t)stages, 0);
   size_t desc_count = 0;
-  for (auto& v : per_job) desc_count += v.size();
-  const size_t bytes = tmp_rows * 32 + desc_count * sizeof(NttPassDesc) + 256;
-  plan->mem = dev_alloc(bytes, stream);
+  const size_t tmp_bytes = (tmp_rows * 32 + 255) & ~(size_t)255;
+  auto plan = new BatchPlan();
+  std::vector<std::vector<NttPassDesc>> per;  // filled below once the scratch base is known
+  for (const TransformSpec& t : specs) {
+    per.push_back(pass_descs(t.in, t.out, nullptr, t.log_n, t.tw, t.pre, t.post, t.post_table, t.q));
+    desc_count += per.back().size();
+  }
+  plan->bytes = tmp_bytes + desc_count * sizeof(NttPassDesc) + 256;
+  plan->mem = dev_alloc(plan->bytes, stream);
   if (!plan->mem) {
     delete plan;
     return 1;
   }
   uint64_t* tmp = static_cast<uint64_t*>(plan->mem);
-  NttPassDesc* dtab = reinterpret_cast<NttPassDesc*>(static_cast<char*>(plan->mem) + ((tmp_rows * 32 + 255) & ~(size_t)255));
-  size_t tmp_at = 0;
-  for (int j = 0; j < count; ++j) {
-    auto& v = per_job[(size_t)j];
+  NttPassDesc* dtab = reinterpret_cast<NttPassDesc*>(static_cast<char*>(plan->mem) + tmp_bytes);
+  for (size_t j = 0; j < specs.size(); ++j) {
+    const TransformSpec& t = specs[j];
+    auto& v = per[j];
+    auto& st = launches[(size_t)t.stage];
     if (v.size() > 1) {
+      uint64_t* mine = tmp + 4 * tmp_at[(size_t)t.stage];
       for (size_t p = 0; p < v.size(); ++p) {
-        if (p > 0) v[p].src = tmp + tmp_at;
-        if (p + 1 < v.size()) v[p].dst = tmp + tmp_at;
+        if (p > 0) v[p].src = mine;
+        if (p + 1 < v.size()) v[p].dst = mine;
       }
-      tmp_at += ((size_t)1 << log_n[j]) * 4;
+      tmp_at[(size_t)t.stage] += (size_t)1 << t.log_n;
     }
-    for (size_t p = 0; p < v.size(); ++p) plan->passes[p].push_back(v[p]);
+    if (st.size() < v.size()) st.resize(v.size());
+    for (size_t p = 0; p < v.size(); ++p) st[p].push_back(v[p]);
   }
+  plan->host_desc.resize(desc_count * sizeof(NttPassDesc));
+  NttPassDesc* host = reinterpret_cast<NttPassDesc*>(plan->host_desc.data());
   size_t at = 0;
-  for (size_t p = 0; p < max_passes; ++p) {
-    int64_t tiles = 0;
-    for (auto& d : plan->passes[p]) {
-      d.tile_base = tiles;
-      tiles += pass_tiles(d);
+  for (auto& st : launches) {
+    for (auto& ds : st) {
+      int64_t tiles = 0;
+      for (NttPassDesc& d : ds) {
+        d.tile_base = tiles;
+        tiles += pass_tiles(d);
+        host[at + (size_t)(&d - ds.data())] = d;
+      }
+      plan->counts.push_back((int)ds.size());
+      plan->tiles.push_back(tiles);
+      plan->d_desc.push_back(dtab + at);
+      at += ds.size();
     }
-    plan->tiles.push_back(tiles);
-    plan->d_passes.push_back(dtab + at);
-    hipError_t e = hipMemcpyAsync(dtab + at, plan->passes[p].data(), sizeof(NttPassDesc) * plan->passes[p].size(),
-                                  hipMemcpyHostToDevice, stream);
+  }
+  // Stream-ordered upload from the plan's own host copy, which lives as long as the plan.
+  if (desc_count) {
+    hipError_t e = hipMemcpyAsync(dtab, host, desc_count * sizeof(NttPassDesc), hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) {
       dev_free(plan->mem);
       delete plan;
-      return fail_hip(e, "NTT plan upload");
+      return fail_hip(e, "batched transform plan upload");
     }
-    at += plan->passes[p].size();
-  }
-  hipError_t e = hipStreamSynchronize(stream);
-  if (e != hipSuccess) {
-    dev_free(plan->mem);
-    delete plan;
-    return fail_hip(e, "NTT plan upload");
   }
   *result = plan;
   return 0;
 }
 
-int ntt_plan_launch(NttPlan* plan, hipStream_t stream) {
+int batch_plan_launch(const BatchPlan* plan, hipStream_t stream) {
   const FrConst f;
-  for (size_t p = 0; p < plan->passes.size(); ++p) {
+  for (size_t i = 0; i < plan->counts.size(); ++i) {
     Prof prof("ntt_pass_multi", stream);
-    hipLaunchKernelGGL((k_ntt_pass_multi<FrConst>), dim3((unsigned)plan->tiles[p]), dim3(kThreads), 0, stream, f,
-                       plan->d_passes[p], (int)plan->passes[p].size());
+    hipLaunchKernelGGL((k_ntt_pass_multi<FrConst>), dim3((unsigned)plan->tiles[i]), dim3(kThreads), 0, stream, f,
+                       plan->d_desc[i], plan->counts[i]);
     ZK_LAUNCH_CHECK("batched ntt pass");
   }
   return 0;
 }
 
-void ntt_plan_destroy(NttPlan* plan) {
+void batch_plan_destroy(BatchPlan* plan) {
   if (!plan) return;
   dev_free(plan->mem);
+  for (void* p : plan->owned) dev_free(p);
   delete plan;
+}
+
+int ntt_plan_create(int count, const uint64_t* const* in, uint64_t* const* out, const int32_t* log_n,
+                    const int32_t* inverse, hipStream_t stream, BatchPlan** result) {
+  std::vector<TransformSpec> specs;
+  const uint64_t one[4] = {1, 0, 0, 0};
+  for (int j = 0; j < count; ++j) {
+    if (log_n[j] < 1 || log_n[j] > 28) return fail("batched NTT sizes must be 2^1..2^28");
+    U256 w, wi;
+    fr_roots(log_n[j], &w, &wi);
+    TransformSpec t;
+    t.in = in[j];
+    t.out = out[j];
+    t.log_n = log_n[j];
+    if (int rc = twiddles(0, log_n[j], inverse[j] ? wi.w : w.w, stream, &t.tw)) return rc;
+    if (inverse[j]) {
+      if (int rc = scale_row(0, log_n[j], one, stream, &t.post)) return rc;
+    }
+    specs.push_back(t);
+  }
+  return batch_plan_create(specs, stream, result);
 }
 
 }  // namespace zk
@@ -523,18 +533,31 @@ extern "C" {
 
 int zk_ntt_cache_clear(void) { return ntt_cache_clear(); }
 
+int zk_ntt_set_fusion(int log_n, int max_stages) {
+  if (log_n < 1 || log_n > 28) return fail("log_n must be in [1, 28]");
+  if (max_stages < 0 || max_stages > 10) return fail("stages per pass must be in 1..10 (0 = default)");
+  g_fusion[log_n] = max_stages;
+  return 0;
+}
+
+int zk_ntt_get_fusion(int log_n, int* max_stages) {
+  if (log_n < 1 || log_n > 28) return fail("log_n must be in [1, 28]");
+  *max_stages = g_fusion[log_n] > 0 ? g_fusion[log_n] : 8;
+  return 0;
+}
+
 int zk_ntt_plan_create(int count, const uint64_t* const* in, uint64_t* const* out, const int32_t* log_n,
                        const int32_t* inverse, void** plan, void* stream) {
-  NttPlan* p = nullptr;
+  BatchPlan* p = nullptr;
   if (int rc = ntt_plan_create(count, in, out, log_n, inverse, (hipStream_t)stream, &p)) return rc;
   *plan = p;
   return 0;
 }
 
-int zk_ntt_plan_launch(void* plan, void* stream) { return ntt_plan_launch(static_cast<NttPlan*>(plan), (hipStream_t)stream); }
+int zk_ntt_plan_launch(void* plan, void* stream) { return batch_plan_launch(static_cast<BatchPlan*>(plan), (hipStream_t)stream); }
 
 int zk_ntt_plan_destroy(void* plan) {
-  ntt_plan_destroy(static_cast<NttPlan*>(plan));
+  batch_plan_destroy(static_cast<BatchPlan*>(plan));
   return 0;
 }
 

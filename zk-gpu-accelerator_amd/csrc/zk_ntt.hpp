@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace zk {
 
 // When a is set, the first pass loads (a*b - c) * z_inv instead of its source
@@ -31,5 +33,45 @@ int ntt_cache_clear();
 // (n rows if post_table, else one row) multiplies the output.
 int ntt_run(int field_id, const uint64_t* in, uint64_t* out, int log_n, const uint64_t* tw, const uint64_t* pre,
             const uint64_t* post, bool post_table, const QuotientArgs& q, hipStream_t stream);
+
+// One transform of a batched launch sequence. Transforms of stage s start after every
+// transform of stage s - 1 has finished; within a stage they run side by side, one launch
+// per pass index.
+struct TransformSpec {
+  const uint64_t* in = nullptr;
+  uint64_t* out = nullptr;
+  int log_n = 0;
+  const uint64_t* tw = nullptr;
+  const uint64_t* pre = nullptr;
+  const uint64_t* post = nullptr;
+  bool post_table = false;
+  QuotientArgs q;
+  int stage = 0;
+};
+
+struct NttPassDesc;
+
+// The launches of a set of transforms with their device descriptor tables and pass scratch,
+// allocated once; launching allocates nothing. `owned` holds further device blocks (from
+// dev_alloc) that are released with the plan.
+struct BatchPlan {
+  std::vector<int> counts;           // descriptors per launch
+  std::vector<int64_t> tiles;        // workgroups per launch
+  std::vector<NttPassDesc*> d_desc;  // device descriptors per launch
+  std::vector<char> host_desc;       // host copy of every descriptor (source of the upload)
+  void* mem = nullptr;
+  std::vector<void*> owned;
+  size_t bytes = 0;                  // device bytes held, owned blocks included
+};
+
+// Scratch rows one transform of size 2^log_n needs between its passes (0 for a single pass).
+int64_t transform_scratch_rows(int log_n);
+int batch_plan_create(const std::vector<TransformSpec>& specs, hipStream_t stream, BatchPlan** out);
+int batch_plan_launch(const BatchPlan* plan, hipStream_t stream);
+void batch_plan_destroy(BatchPlan* plan);
+
+// The standard primitive 2^log_n-th root of unity of Fr (generator 5) and its inverse, plain.
+struct U256;
+void fr_roots(int log_n, U256* omega, U256* omega_inv);
 
 }  // namespace zk

@@ -58,13 +58,16 @@ static void release_cache_locked() {
 void* dev_alloc(size_t bytes, hipStream_t stream) {
   size_t c = size_class(bytes);
   {
+    // the smallest cached block of this stream that holds the request and is at most twice its size
     std::lock_guard<std::mutex> lk(g_alloc_mu);
-    auto it = g_free.find({stream, c});
-    if (it != g_free.end() && !it->second.empty()) {
+    for (auto it = g_free.lower_bound({stream, c}); it != g_free.end() && it->first.first == stream &&
+                                                    it->first.second <= 2 * c;
+         ++it) {
+      if (it->second.empty()) continue;
       void* p = it->second.back();
       it->second.pop_back();
-      g_cached_bytes -= c;
-      g_live[p] = Block{c, stream};
+      g_cached_bytes -= it->first.second;
+      g_live[p] = Block{it->first.second, stream};
       return p;
     }
   }
@@ -313,6 +316,25 @@ int zk_event_record(void* event, void* stream) {
 int zk_event_elapsed_ms(void* start, void* end, float* ms) {
   ZK_HIP(hipEventSynchronize((hipEvent_t)end));
   ZK_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+  return 0;
+}
+
+int zk_event_query(void* event, int* done) {
+  hipError_t e = hipEventQuery((hipEvent_t)event);
+  if (e == hipSuccess) {
+    *done = 1;
+    return 0;
+  }
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();
+    *done = 0;
+    return 0;
+  }
+  return fail_hip(e, "hipEventQuery");
+}
+
+int zk_event_sync(void* event) {
+  ZK_HIP(hipEventSynchronize((hipEvent_t)event));
   return 0;
 }
 

@@ -3,7 +3,8 @@ zkgpu.batch: many MSMs and NTTs of uneven sizes as one launch sequence.
 
 SRS holds a device-resident basis, converted once into the kernels' internal
 layout (the reference converts and uploads its points on every call,
-src/zkgpu/msm_gpu.py:348). MsmBatch plans a set of MSMs over prefixes of an
+src/zkgpu/msm_gpu.py:348). QuotientBatch runs the Groth16 quotient of several
+jobs of mixed sizes as one launch sequence. MsmBatch plans a set of MSMs over prefixes of an
 SRS: one digit sort, one load-balanced bucket accumulation, one bucket
 reduction and one window combination for all of them together, each MSM with
 its own window width. NttBatch runs several Fr NTTs of different sizes as one
@@ -60,12 +61,15 @@ class SRS:
 
 
 def _tuned_window(n: int) -> int:
-    try:
-        from zkgpu.autotune import lookup
+    from zkgpu.autotune import lookup
 
-        return lookup("msm_window", n) or 0
-    except Exception:
-        return 0
+    return lookup("msm_window", n)
+
+
+def _tuned_chunk(sizes: Sequence[int]) -> int:
+    from zkgpu.autotune import lookup
+
+    return lookup("msm_chunk", max(sizes)) if sizes else 0
 
 
 class MsmBatch:
@@ -75,7 +79,8 @@ class MsmBatch:
     a (count, 3, 4) Jacobian device array; results() converts them to Points.
     """
 
-    def __init__(self, srs: SRS, scalars: Sequence[DeviceArray], *, windows: Sequence[int] | None = None, chunk_len: int = 0) -> None:
+    def __init__(self, srs: SRS, scalars: Sequence[DeviceArray], *, windows: Sequence[int] | None = None,
+                 chunk_len: int | None = None) -> None:
         _gb.require_gpu("MsmBatch")
         self.srs = srs
         self.scalars = [_gb.asarray(s) for s in scalars]
@@ -84,6 +89,8 @@ class MsmBatch:
             raise ValueError(f"an MSM is larger than the SRS ({srs.n} points)")
         count = len(self.scalars)
         self.windows = list(windows) if windows is not None else [_tuned_window(n) for n in self.sizes]
+        if chunk_len is None:
+            chunk_len = _tuned_chunk(self.sizes)
         self.out = _gb.empty((max(count, 1), 3, 4))
         h = ctypes.c_void_p()
         sizes = (ctypes.c_int64 * max(count, 1))(*self.sizes)
@@ -140,6 +147,68 @@ class NttBatch:
         if _gb.LIB is not None and getattr(self, "handle", None):
             _gb.LIB.zk_ntt_plan_destroy(self.handle)
             self.handle = None
+
+
+class QuotientBatch:
+    """
+    The Groth16 quotient h_j = (A_j B_j - C_j) / Z_H on the coset g_j H for several jobs of mixed
+    sizes (Radix2Domain.quotient, one launch per stage and pass index for all of them). a, b, c
+    are (n_j, 4) Montgomery rows on the device; results land in h (allocated here unless given).
+    """
+
+    def __init__(self, a: Sequence[DeviceArray], b: Sequence[DeviceArray], c: Sequence[DeviceArray], *,
+                 shifts: Sequence[int] | int = 7, h: Sequence[DeviceArray] | None = None) -> None:
+        _gb.require_gpu("QuotientBatch")
+        if not len(a) == len(b) == len(c):
+            raise ValueError("a, b and c must have the same length")
+        self.a, self.b, self.c = ([_gb.asarray(x) for x in v] for v in (a, b, c))
+        count = len(self.a)
+        log_n = []
+        for x, y, z in zip(self.a, self.b, self.c):
+            n = int(x.shape[0])
+            if n < 2 or n & (n - 1) or tuple(x.shape) != (n, 4) or tuple(y.shape) != (n, 4) or tuple(z.shape) != (n, 4):
+                raise ValueError("quotient sizes must be powers of two >= 2 with (n, 4) rows for a, b and c")
+            log_n.append(n.bit_length() - 1)
+        self.log_n = log_n
+        if isinstance(shifts, int):
+            shifts = [shifts] * count
+        if len(shifts) != count:
+            raise ValueError("one coset shift per job")
+        self.h = list(h) if h is not None else [_gb.empty((1 << k, 4)) for k in log_n]
+        shift_rows = np.zeros((max(count, 1), 4), dtype=np.uint64)
+        for j, g in enumerate(shifts):
+            shift_rows[j] = [(int(g) >> (64 * k)) & (2**64 - 1) for k in range(4)]
+        hnd = ctypes.c_void_p()
+        _gb.call(
+            "zk_groth16_plan_create", count, _ptr_array([x.ptr for x in self.a]), _ptr_array([x.ptr for x in self.b]),
+            _ptr_array([x.ptr for x in self.c]), _ptr_array([x.ptr for x in self.h]),
+            (ctypes.c_int32 * max(count, 1))(*log_n), shift_rows.ctypes.data, ctypes.byref(hnd), _gb.current_stream(),
+        )
+        self.handle = hnd.value
+
+    def launch(self) -> None:
+        _gb.call("zk_groth16_plan_launch", self.handle, _gb.current_stream())
+
+    def __del__(self) -> None:
+        if _gb.LIB is not None and getattr(self, "handle", None):
+            _gb.LIB.zk_groth16_plan_destroy(self.handle)
+            self.handle = None
+
+
+def quotient_workspace_bytes(log_n: Sequence[int]) -> int:
+    """Device bytes a QuotientBatch over these sizes holds (besides its inputs and outputs)."""
+    out = ctypes.c_uint64()
+    _gb.call("zk_groth16_workspace_bytes", len(log_n), (ctypes.c_int32 * max(len(log_n), 1))(*log_n), ctypes.byref(out))
+    return int(out.value)
+
+
+def msm_workspace_bytes(sizes: Sequence[int], windows: Sequence[int] | None = None) -> int:
+    """Device bytes an MsmBatch over these sizes holds (besides its scalars and results)."""
+    out = ctypes.c_uint64()
+    count = len(sizes)
+    wins = (ctypes.c_int32 * max(count, 1))(*(windows if windows is not None else [_tuned_window(n) for n in sizes]))
+    _gb.call("zk_msm_workspace_bytes", count, (ctypes.c_int64 * max(count, 1))(*sizes), wins, 0, ctypes.byref(out))
+    return int(out.value)
 
 
 def capture(work: Sequence[Any], on: _gb.Stream) -> _gb.Graph:

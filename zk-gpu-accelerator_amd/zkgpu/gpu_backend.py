@@ -75,6 +75,8 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_event_destroy": [_vp],
     "zk_event_record": [_vp, _vp],
     "zk_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
+    "zk_event_query": [_vp, ctypes.POINTER(_i32)],
+    "zk_event_sync": [_vp],
     "zk_stream_wait_event": [_vp, _vp],
     "zk_selftest": [],
     "zk_graph_begin": [_vp],
@@ -93,7 +95,13 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_field_sub": [_i32, _vp, _i32, _vp, _i32, _vp, _i64, _vp],
     "zk_ntt": [_i32, _vp, _vp, _i32, _i32, _u64p, _u64p, _vp],
     "zk_ntt_cache_clear": [],
+    "zk_ntt_set_fusion": [_i32, _i32],
+    "zk_ntt_get_fusion": [_i32, ctypes.POINTER(_i32)],
     "zk_groth16_quotient": [_vp, _vp, _vp, _vp, _i32, _u64p, _vp],
+    "zk_groth16_plan_create": [_i32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp],
+    "zk_groth16_plan_launch": [_vp, _vp],
+    "zk_groth16_plan_destroy": [_vp],
+    "zk_groth16_workspace_bytes": [_i32, _vp, ctypes.POINTER(_u64)],
     "zk_g1_add": [_vp, _i32, _vp, _i32, _vp, _i64, _vp],
     "zk_g1_add_affine": [_vp, _i32, _vp, _i32, _vp, _i64, _vp],
     "zk_g1_double": [_vp, _vp, _i64, _vp],
@@ -105,6 +113,7 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_msm_plan_create": [_vp, _i32, _vp, _vp, _vp, _i32, _vp, ctypes.POINTER(_vp), _vp],
     "zk_msm_plan_launch": [_vp, _vp],
     "zk_msm_plan_destroy": [_vp],
+    "zk_msm_workspace_bytes": [_i32, _vp, _vp, _i32, ctypes.POINTER(_u64)],
     "zk_msm_g1_batch": [_vp, _i32, _vp, _vp, _vp, _vp, _vp],
     "zk_ntt_batch": [_i32, _vp, _vp, _vp, _vp, _vp],
     "zk_ntt_plan_create": [_i32, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp],
@@ -227,6 +236,15 @@ class Event:
         out = ctypes.c_float()
         call("zk_event_elapsed_ms", self.handle, end.handle, ctypes.byref(out))
         return float(out.value)
+
+    def done(self) -> bool:
+        """True once all work recorded before this event has finished (never blocks)."""
+        out = ctypes.c_int()
+        call("zk_event_query", self.handle, ctypes.byref(out))
+        return bool(out.value)
+
+    def synchronize(self) -> None:
+        call("zk_event_sync", self.handle)
 
     def __del__(self) -> None:
         if LIB is not None and getattr(self, "handle", None):
@@ -468,3 +486,14 @@ def u64_array(values: Sequence[int]) -> Any:
 def limbs_of(value: int) -> Any:
     """ctypes uint64[4] of a value below 2^256, little-endian."""
     return u64_array([(value >> (64 * j)) & (2**64 - 1) for j in range(4)])
+
+
+def _apply_tuning() -> None:
+    """Hand this GPU's autotuned NTT fusion depths (zkgpu/tuning/<device>.json) to the library."""
+    if GPU_AVAILABLE:
+        from zkgpu import autotune
+
+        autotune.apply()
+
+
+_apply_tuning()

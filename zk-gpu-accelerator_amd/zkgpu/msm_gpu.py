@@ -83,8 +83,16 @@ def _result(jac_rows: DeviceArray) -> Point:
     return from_jacobian((x, y, z))
 
 
+def _tuned(n: int, window: int, chunk: int) -> tuple[int, int]:
+    """0 means automatic: this GPU's autotuned window and accumulation chunk, else the cost model."""
+    from zkgpu.autotune import lookup
+
+    return (window or lookup("msm_window", n)), (chunk or lookup("msm_chunk", n))
+
+
 def _run(points: DeviceArray, scalars: DeviceArray, n: int, window: int, chunk: int) -> DeviceArray:
     out = _gb.empty((3, _FIELD_LIMBS))
+    window, chunk = _tuned(n, window, chunk)
     _gb.call("zk_msm_g1", points.ptr, scalars.ptr, n, window, chunk, out.ptr, _gb.current_stream())
     return out
 
@@ -120,7 +128,8 @@ def msm_pippenger_gpu(
     ----------
     points_x, points_y : affine coordinates (Python ints or an object array)
     scalars            : any integers; reduced mod r, which leaves every s_i * P_i unchanged
-    window_size        : bits per signed digit, 1..16 (default: the device cost model)
+    window_size        : bits per signed digit, 1..16 (default: this GPU's autotuned
+                         window, else the device cost model)
 
     Raises
     ------
@@ -168,4 +177,6 @@ def msm_g1(points: Any, scalars: Any, *, window_size: int | None = None) -> Poin
 
 def msm_g1_jacobian(points: DeviceArray, scalars: DeviceArray, out: DeviceArray, *, window_size: int = 0, chunk_len: int = 0) -> None:
     """Enqueue an MSM whose Jacobian result (3, 4) Montgomery rows stays on the device in out."""
-    _gb.call("zk_msm_g1", points.ptr, scalars.ptr, points.shape[0], window_size, chunk_len, out.ptr, _gb.current_stream())
+    n = points.shape[0]
+    window_size, chunk_len = _tuned(n, window_size, chunk_len)
+    _gb.call("zk_msm_g1", points.ptr, scalars.ptr, n, window_size, chunk_len, out.ptr, _gb.current_stream())

@@ -7,6 +7,8 @@ Seeded synthetic inputs for the benchmark configurations (no datasets exist for 
   points are computed on the GPU by zk_g1_fixed_base.
 - witness_scalars: the witness-like mix per index, u = rng.random():
   u < 0.5 uniform in [0, r), u < 0.8 uniform in {0, 1}, else uniform below 2^32.
+- witness_rows_fast: the same mix drawn with numpy (device pools of the
+  served workload).
 - log_uniform_sizes: sizes 2^U with U uniform in [lo, hi] for the batched and
   served workloads.
 All generators return plain little-endian (n, 4) uint64 limbs unless noted.
@@ -70,6 +72,20 @@ def witness_scalars(n: int, seed: int = 0x5EED02) -> list[int]:
 
 def witness_rows(n: int, seed: int = 0x5EED02) -> np.ndarray:
     return _limbs(witness_scalars(n, seed))
+
+
+def witness_rows_fast(n: int, seed: int) -> np.ndarray:
+    """The witness-like mix as plain rows from numpy's generator (for large device pools)."""
+    gen = np.random.default_rng(seed)
+    u = gen.random(n)
+    out = uniform_rows(n, seed ^ 0x9E3779B9)
+    small = u >= 0.5
+    out[small] = 0
+    boolean = small & (u < 0.8)
+    out[boolean, 0] = gen.integers(0, 2, size=int(boolean.sum()), dtype=np.uint64)
+    word = u >= 0.8
+    out[word, 0] = gen.integers(0, 1 << 32, size=int(word.sum()), dtype=np.uint64)
+    return out
 
 
 def key_rows(n: int, seed: int = 0x5EED02) -> np.ndarray:
