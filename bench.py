@@ -10,14 +10,16 @@ quotient pipeline at 2^22, the batched form against the plain per-call path
 on the same mixed-size workload, and the served form's p50/p99 latency, and
 prints ONE JSON line.
 
-roofline: the dominant kernel of a step is the bucket accumulation
-(msm_accumulate); its duration comes from HIP events on its own stream around
-each launch (zk_prof_*). Algorithmic work per launch = (nonzero signed
-digits) x 11 Fq multiplications x 136 32-bit multiply-adds, the cost basis of
-a mixed addition on 8x32-bit limbs; the peak is gfx950's measured
-v_mad_u64_u32 rate (zk_valu_peak). cpu_baseline: the oracle's Python-int
-msm_pippenger (the reference's execution model: one core) on a 2^18-point
-sample of the same workload.
+roofline: the bucket accumulation (k_accumulate) is the step's main
+throughput kernel; its duration comes from HIP events recorded on its own
+stream around each launch (zk_prof_*). Algorithmic work per launch = (nonzero
+signed digits) x 11 Fq multiplications x 136 32-bit multiply-adds, the cost
+basis of a mixed addition on 8x32-bit limbs; the peak is gfx950's measured
+v_mad_u64_u32 rate (zk_valu_peak). Its HBM traffic per launch comes from the
+committed byte-counter summary profiles/k_accumulate_traffic.json (rocprofv3
+FETCH_SIZE and WRITE_SIZE runs scaled by the zk_copy probe, --traffic-probe).
+cpu_baseline: the oracle's Python-int msm_pippenger (the reference's execution
+model: one core) on a 2^18-point sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -88,6 +90,8 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=300.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serve-budget-gib", type=float, default=24.0, help="HBM budget of the server")
     ap.add_argument("--plain-sample", type=int, default=200, help="jobs timed through the per-call path")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="copy 256 MiB with zk_copy first (the known-size kernel for scaling byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -118,6 +122,12 @@ def main() -> int:
     def msm_step() -> None:
         msm_g1_jacobian(points, scalars, out, window_size=window)
 
+    if args.traffic_probe:
+        probe = 256 << 20
+        a, b = gb.empty(probe // 8), gb.empty(probe // 8)
+        gb.call("zk_copy", b.ptr, a.ptr, probe, gb.current_stream())
+        gb.synchronize()
+        del a, b
     log("msm: warmup + timed")
     gb.prof_enable(True)
     for _ in range(args.warmup):
@@ -145,6 +155,7 @@ def main() -> int:
     log(f"msm 2^{args.log_msm}: {ms:.3f} ms/step, {msm_mpts:.2f} Mpts/s; accumulate {acc_avg:.3f} ms "
         f"({achieved_tmads:.2f} Tmad/s of {peak_tmads:.2f}); final {fin_ms / max(fin_launches, 1):.3f} ms")
 
+    traffic = _traffic("k_accumulate", n)
     extras: dict = {}
     if not args.no_extras:
         extras.update(_ntt_and_pipeline(args, gb, BN254_FR, workloads, log))
@@ -173,22 +184,35 @@ def main() -> int:
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": "msm_accumulate",
+            "kernel": "k_accumulate",
             "bound": "valu",
             "achieved": round(achieved_tmads, 4),
             "peak": round(peak_tmads, 4),
             "unit": "Tmad/s",
             "frac": round(achieved_tmads / peak_tmads, 4) if peak_tmads else None,
-            "traffic": None,
+            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "traffic_source": traffic.get("source") if traffic else None,
+            "algorithmic_bytes_per_launch": entries * 72,
             "avg_launch_ms": round(acc_avg, 4),
             "units_per_launch": entries,
-            "work_per_unit": "11 Fq mul x 136 32-bit mad (one mixed addition)",
+            "work_per_unit": "one mixed addition: 11 Fq mul x 136 32-bit mad; 72 bytes (key, point ref, affine point)",
         },
         "cpu_baseline": cpu,
         **extras,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _traffic(kernel: str, n: int) -> dict | None:
+    """HBM bytes per launch of `kernel` from the committed byte-counter summary, if one exists."""
+    path = ROOT / "profiles" / f"{kernel}_traffic.json"
+    if not path.exists():
+        return None
+    data = json.loads(path.read_text())
+    if data.get("msm_points") != n:
+        return None
+    return {"bytes_per_launch": data["bytes_per_launch"], "source": f"profiles/{path.name}"}
 
 
 def _default_window(gb, n: int) -> int:
@@ -228,10 +252,15 @@ def _ntt_and_pipeline(args, gb, Fr, workloads, log) -> dict:
     pass_ms, passes = gb.prof_read("ntt_pass", reset=True)
     per_pass = pass_ms / max(passes, 1)
     passes_per_ntt = max(1, round(passes / (args.steps + 1)))
+    ntt_pass_ms = per_pass * passes_per_ntt  # kernel time of one transform
     melem = n / (ms * 1e-3) / 1e6
     hbm_gbps = 64.0 * n / (ms * 1e-3) / 1e9
     pass_gbps = 64.0 * n / (per_pass * 1e-3) / 1e9  # each pass reads and writes every row once
-    log(f"ntt 2^{args.log_ntt}: {ms:.3f} ms, {melem:.1f} Melem/s; pass {per_pass:.3f} ms ({pass_gbps:.0f} GB/s)")
+    # one Montgomery product per butterfly, n/2 butterflies per stage, log n stages
+    ntt_tmads = args.log_ntt * (n // 2) * 136 / (ntt_pass_ms * 1e-3) / 1e12
+    peak = _valu_peak(gb)
+    log(f"ntt 2^{args.log_ntt}: {ms:.3f} ms, {melem:.1f} Melem/s; pass {per_pass:.3f} ms ({pass_gbps:.0f} GB/s, "
+        f"{ntt_tmads:.2f} Tmad/s of {peak:.2f})")
 
     log("groth16 quotient pipeline at the same size (7 NTTs)")
     a, b, c = (Fr.from_limbs(workloads.uniform_rows(n, s)) for s in (0x5EED31, 0x5EED32, 0x5EED33))
@@ -253,12 +282,15 @@ def _ntt_and_pipeline(args, gb, Fr, workloads, log) -> dict:
             "pass_ms": round(per_pass, 4),
             "passes": passes_per_ntt,
             "roofline": {
-                "kernel": "ntt_pass",
-                "bound": "hbm",
-                "achieved": round(pass_gbps, 1),
-                "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s",
-                "frac": round(pass_gbps / HBM_PEAK_GBPS, 4),
+                "kernel": "k_ntt_pass",
+                "bound": "valu",
+                "achieved": round(ntt_tmads, 4),
+                "peak": round(peak, 4),
+                "unit": "Tmad/s",
+                "frac": round(ntt_tmads / peak, 4) if peak else None,
+                "work_per_unit": "one butterfly: 1 Fr mul x 136 32-bit mad; n/2 butterflies x log n stages",
+                "hbm": {"achieved": round(pass_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": round(pass_gbps / HBM_PEAK_GBPS, 4), "bytes_per_pass": 64 * n},
                 "traffic": None,
             },
         },

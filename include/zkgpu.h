@@ -47,6 +47,9 @@ int zk_memcpy_h2d(void* dst, const void* host_src, uint64_t bytes, void* stream)
 int zk_memcpy_d2h(void* host_dst, const void* src, uint64_t bytes, void* stream);  /* synchronises stream */
 int zk_memcpy_d2d(void* dst, const void* src, uint64_t bytes, void* stream);
 int zk_memset(void* dst, int value, uint64_t bytes, void* stream);
+/* A plain copy kernel (each byte read and written once; 16-byte aligned buffers and sizes): the
+ * known-size reference that hardware byte counters are scaled by. No counterpart in the reference. */
+int zk_copy(void* dst, const void* src, uint64_t bytes, void* stream);
 int zk_stream_create(void** stream);
 int zk_stream_destroy(void* stream);
 int zk_stream_sync(void* stream);

@@ -9,14 +9,15 @@
 //              buckets per window against the reference's unsigned ones
 //              (src/zkgpu/msm_gpu.py:89-105).
 //  3. sort     counting sort of the (window, bucket) keys with no global atomics:
-//              digits are written window-major, one workgroup per (window, range
+//              digits are written window-major; one workgroup per (window, range
 //              of 16384 buckets, tile of scalars) counts its tile's digits into an
-//              LDS histogram, an exclusive scan over the (bucket, tile) counts gives
-//              every tile its first position inside each bucket, and the same
-//              workgroups then place (key, point index | sign) pairs from LDS
-//              cursors. Witness-like scalars put a large share of all entries into
-//              a few buckets (booleans send ~15% of them to bucket 1 of window 0);
-//              those hot keys only ever meet LDS atomics.
+//              LDS histogram and writes it as one contiguous row; one pass down
+//              the tiles of each bucket turns the counts into per-tile offsets and
+//              a bucket total, a scan of the totals gives every bucket its start,
+//              and the same workgroups then place packed (key, point index | sign)
+//              records from LDS cursors. Witness-like scalars put a large share of
+//              all entries into a few buckets (booleans send ~15% of them to
+//              bucket 1 of window 0); those hot keys only ever meet LDS atomics.
 //  4. buckets  a persistent grid splits the m sorted entries evenly, ceil(m / threads)
 //              consecutive entries per thread, added with mixed Jacobian + affine
 //              additions; a run of equal keys strictly inside a thread's range is a
@@ -41,6 +42,7 @@
 
 #include "../../include/zkgpu.h"
 #include "zk_curve.hpp"
+#include "zk_curve_wave.hpp"
 #include "zk_fields.hpp"
 #include "zk_msm.hpp"
 #include "zk_scan.hpp"
@@ -237,9 +239,9 @@ __device__ __forceinline__ SortBlock sort_block(const MsmJob* __restrict__ jobs,
   return b;
 }
 
-// Counts, per (bucket, tile), the digits of one tile that fall into this block's bucket range;
-// counts are laid out bucket-major, tile-minor, so their exclusive scan gives each
-// (bucket, tile) its first sorted position.
+// Counts, per (bucket, tile), the digits of one tile that fall into this block's bucket range.
+// Counts are laid out tile-major (window w, tile t: a row of 2^(c-1) buckets), so every
+// workgroup writes its range as one contiguous run.
 __global__ void __launch_bounds__(256) k_sort_count(const MsmJob* __restrict__ jobs, int njobs,
                                                     const int32_t* __restrict__ digits, uint32_t* __restrict__ counts) {
   extern __shared__ uint32_t hist[];
@@ -258,34 +260,54 @@ __global__ void __launch_bounds__(256) k_sort_count(const MsmJob* __restrict__ j
     if (v != 0 && b >= 0 && b < range) atomicAdd(&hist[b], 1u);
   }
   __syncthreads();
-  uint32_t* out = counts + job.cnt_base + ((int64_t)sb.win * buckets + r0) * job.tiles + sb.tile;
-  for (int k = threadIdx.x; k < range; k += 256) out[(int64_t)k * job.tiles] = hist[k];
+  uint32_t* out = counts + job.cnt_base + ((int64_t)sb.win * job.tiles + sb.tile) * buckets + r0;
+  for (int k = threadIdx.x; k < range; k += 256) out[k] = hist[k];
 }
 
-// Places each digit's (key, point ref) at its bucket's next free position: the block's LDS
+// One thread per bucket key: turns the bucket's per-tile counts into per-tile offsets inside the
+// bucket (an exclusive scan down the tiles, coalesced across neighbouring buckets) and writes
+// the bucket's total, whose scan over all keys gives every bucket its first sorted position.
+__global__ void __launch_bounds__(256) k_sort_columns(const MsmJob* __restrict__ jobs, int njobs, int64_t nkeys,
+                                                      uint32_t* __restrict__ counts, uint32_t* __restrict__ totals) {
+  const int64_t key = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (key >= nkeys) return;
+  const MsmJob& job = jobs[find_job<&MsmJob::key_base>(jobs, njobs, key)];
+  const int64_t local = key - job.key_base;
+  const int64_t buckets = 1LL << (job.c - 1);
+  const int64_t win = local >> (job.c - 1), b = local & (buckets - 1);
+  uint32_t* col = counts + job.cnt_base + win * job.tiles * buckets + b;
+  uint32_t run = 0;
+  for (int t = 0; t < job.tiles; ++t) {
+    const uint32_t v = col[(int64_t)t * buckets];
+    col[(int64_t)t * buckets] = run;
+    run += v;
+  }
+  totals[key] = run;
+}
+
+// Places each digit's (key, point ref) record at its bucket's next free position: the block's LDS
 // holds the running positions of its (bucket, tile) slots, so no global atomics are needed.
 __global__ void __launch_bounds__(256) k_sort_place(const MsmJob* __restrict__ jobs, int njobs,
-                                                    const int32_t* __restrict__ digits, const uint32_t* __restrict__ offsets,
-                                                    uint32_t* __restrict__ skey, uint32_t* __restrict__ sref) {
+                                                    const int32_t* __restrict__ digits, const uint32_t* __restrict__ counts,
+                                                    const uint32_t* __restrict__ starts, uint2* __restrict__ sorted) {
   extern __shared__ uint32_t cur[];
   const SortBlock sb = sort_block(jobs, njobs);
   const MsmJob& job = *sb.job;
   const int64_t buckets = 1LL << (job.c - 1);
   const int64_t r0 = (int64_t)sb.range * kSortRange;
   const int range = (int)min((int64_t)kSortRange, buckets - r0);
-  const uint32_t* off = offsets + job.cnt_base + ((int64_t)sb.win * buckets + r0) * job.tiles + sb.tile;
-  for (int k = threadIdx.x; k < range; k += 256) cur[k] = off[(int64_t)k * job.tiles];
+  const uint32_t* off = counts + job.cnt_base + ((int64_t)sb.win * job.tiles + sb.tile) * buckets + r0;
+  const int64_t key0 = job.key_base + (int64_t)sb.win * buckets + r0;
+  for (int k = threadIdx.x; k < range; k += 256) cur[k] = starts[key0 + k] + off[k];
   __syncthreads();
   const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.n, lo + job.tile);
   const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.n;
-  const uint32_t key0 = (uint32_t)(job.key_base + (int64_t)sb.win * buckets + r0);
   for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
     const int32_t v = d[i];
     const int64_t b = (int64_t)(v < 0 ? -v : v) - 1 - r0;
     if (v != 0 && b >= 0 && b < range) {
       const uint32_t pos = atomicAdd(&cur[b], 1u);
-      skey[pos] = key0 + (uint32_t)b;
-      sref[pos] = (uint32_t)i | (v < 0 ? 0x80000000u : 0u);
+      sorted[pos] = make_uint2((uint32_t)(key0 + b), (uint32_t)i | (v < 0 ? 0x80000000u : 0u));
     }
   }
 }
@@ -372,7 +394,7 @@ constexpr int kCombineThreads = 256;
 // Level 0, a persistent grid: the m sorted entries are split evenly, ceil(m / threads) consecutive
 // entries per thread (mixed additions), and each workgroup merges its threads' run summaries.
 __global__ void __launch_bounds__(kAccThreads)
-k_accumulate(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sref, const uint32_t* __restrict__ m_dev,
+k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_dev,
              const AffRec* __restrict__ pts, JacRec* __restrict__ buckets, RunSummary* __restrict__ out, int last_level) {
   __shared__ RunSummary sh[kAccThreads];
   const int64_t threads = (int64_t)gridDim.x * kAccThreads;
@@ -387,12 +409,13 @@ k_accumulate(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sre
   if (s < m) {
     const int64_t e = min(s + k, m);
     Jac acc = jac_identity(f);
-    uint32_t cur = skey[s];
+    uint32_t cur = sorted[s].x;
     bool first = true;
     me.fk = cur;
     me.flags = kValid | kSingle;
     for (int64_t x = s; x <= e; ++x) {
-      const uint32_t key = x < e ? skey[x] : 0xffffffffu;
+      const uint2 rec = x < e ? sorted[x] : make_uint2(0xffffffffu, 0u);
+      const uint32_t key = rec.x;
       if (key != cur) {
         const bool is_last = x == e;
         if (first && is_last) {
@@ -411,7 +434,7 @@ k_accumulate(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sre
         cur = key;
         acc = jac_identity(f);
       }
-      const uint32_t ref = sref[x];
+      const uint32_t ref = rec.y;
       Aff q = load_aff_rec(pts + (ref & 0x7fffffffu));
       if (ref >> 31) q.y = fneg(f, q.y);
       acc = jac_madd(f, acc, q);
@@ -515,10 +538,11 @@ __global__ void __launch_bounds__(64) k_row_sums(const MsmJob* __restrict__ jobs
   store_jac_rec(row_out + g, r);
 }
 
-// One thread per (job, window): U = sum T_s + 2^log_seg * sum_j 2^j Q_j (Horner over j).
+// One wavefront per (job, window): U = sum T_s + 2^log_seg * sum_j 2^j Q_j (Horner over j), with
+// the wave-parallel formulas (the chain is latency-bound).
 __global__ void __launch_bounds__(64) k_window_sums(const MsmJob* __restrict__ jobs, int njobs, int64_t total_windows,
                                                     const JacRec* __restrict__ rows, JacRec* __restrict__ wsum) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t g = blockIdx.x;
   if (g >= total_windows) return;
   const MsmJob& job = jobs[find_job<&MsmJob::win_base>(jobs, njobs, g)];
   const int64_t win = g - job.win_base;
@@ -527,26 +551,26 @@ __global__ void __launch_bounds__(64) k_window_sums(const MsmJob* __restrict__ j
   const FqIlp f;
   Jac acc = jac_identity(f);
   for (int j = job.log_segs - 1; j >= 0; --j) {
-    acc = jac_dbl(f, acc);
-    acc = jac_add(f, acc, load_jac_rec(p + j));
+    acc = jac_dbl_wave(f, acc);
+    acc = jac_add_wave(f, acc, load_jac_rec(p + j));
   }
-  for (int k = 0; k < job.log_seg; ++k) acc = jac_dbl(f, acc);
-  acc = jac_add(f, acc, load_jac_rec(p + job.log_segs));
-  store_jac_rec(wsum + g, acc);
+  for (int k = 0; k < job.log_seg; ++k) acc = jac_dbl_wave(f, acc);
+  acc = jac_add_wave(f, acc, load_jac_rec(p + job.log_segs));
+  if (threadIdx.x == 0) store_jac_rec(wsum + g, acc);
 }
 
-// One thread per job: Horner over the windows, sum_w 2^(cw) U_w.
-__global__ void k_final(const MsmJob* __restrict__ jobs, int njobs, const JacRec* __restrict__ wsum) {
-  const int jb = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per job: Horner over the windows, sum_w 2^(cw) U_w (wave-parallel formulas).
+__global__ void __launch_bounds__(64) k_final(const MsmJob* __restrict__ jobs, int njobs, const JacRec* __restrict__ wsum) {
+  const int jb = blockIdx.x;
   if (jb >= njobs) return;
   const MsmJob& job = jobs[jb];
   const FqIlp f;
   Jac acc = jac_identity(f);
   for (int w = job.windows - 1; w >= 0; --w) {
-    for (int e = 0; e < job.c; ++e) acc = jac_dbl(f, acc);
-    acc = jac_add(f, acc, load_jac_rec(wsum + job.win_base + w));
+    for (int e = 0; e < job.c; ++e) acc = jac_dbl_wave(f, acc);
+    acc = jac_add_wave(f, acc, load_jac_rec(wsum + job.win_base + w));
   }
-  store_jac(f, job.out, acc);
+  if (threadIdx.x == 0) store_jac(f, job.out, acc);
 }
 
 // ---- host ---------------------------------------------------------------------
@@ -573,14 +597,15 @@ struct MsmPlan {
   std::vector<MsmJob> jobs;
   std::vector<int64_t> level_max;
   int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, segments = 0, sums_rows = 0, parts = 0, windows_total = 0;
-  int64_t sort_blocks = 0, ncounts = 0;
+  int64_t sort_blocks = 0;
   size_t bytes = 0;
   int32_t* digits = nullptr;
   const AffRec* points = nullptr;
   // device buffers
   void* mem = nullptr;
   MsmJob* d_jobs = nullptr;
-  uint32_t *counts = nullptr, *offsets = nullptr, *m_dev = nullptr, *skey = nullptr, *sref = nullptr, *scan_ws = nullptr;
+  uint32_t *counts = nullptr, *totals = nullptr, *starts = nullptr, *m_dev = nullptr, *scan_ws = nullptr;
+  uint2* sorted = nullptr;
   JacRec* buckets = nullptr;
   RunSummary* runs[2] = {nullptr, nullptr};
   JacRec *tot = nullptr, *run = nullptr, *parts_buf = nullptr, *rows_buf = nullptr, *wsum = nullptr;
@@ -590,7 +615,7 @@ static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Byte offsets of every buffer inside a plan's single device allocation.
 struct MsmLayout {
-  size_t jobs, counts, offsets, digits, m, skey, sref, scan, buckets, r0, r1, tot, run, parts, rows, wsum, total;
+  size_t jobs, counts, totals, starts, digits, m, sorted, scan, buckets, r0, r1, tot, run, parts, rows, wsum, total;
 };
 
 // jobs: scalars, n, c (0 = default) and out filled in by the caller; the rest is planned here,
@@ -655,7 +680,6 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   plan->segments = segs;
   plan->sums_rows = sums;
   plan->sort_blocks = sort_blocks;
-  plan->ncounts = cnts;
   plan->parts = parts;
   plan->windows_total = wins;
   // Accumulation levels: level 0 leaves one run summary per workgroup; each further level merges
@@ -673,13 +697,13 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
     return at;
   };
   L->jobs = take(sizeof(MsmJob) * jobs.size());
-  L->counts = take(4 * (size_t)(cnts + 1));
-  L->offsets = take(4 * (size_t)(cnts + 1));
+  L->counts = take(4 * (size_t)std::max<int64_t>(cnts, 1));
+  L->totals = take(4 * (size_t)(keys + 1));
+  L->starts = take(4 * (size_t)(keys + 1));
   L->digits = take(4 * e1);
   L->m = take(16);
-  L->skey = take(4 * e1);
-  L->sref = take(4 * e1);
-  L->scan = take(4 * scan_workspace_words(cnts));
+  L->sorted = take(8 * e1);
+  L->scan = take(4 * scan_workspace_words(keys));
   L->buckets = take(sizeof(JacRec) * (size_t)keys);
   L->r0 = take(sizeof(RunSummary) * (size_t)max_slots);
   L->r1 = take(sizeof(RunSummary) * (size_t)max_slots);
@@ -709,11 +733,11 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   char* base = static_cast<char*>(plan->mem);
   plan->d_jobs = reinterpret_cast<MsmJob*>(base + L.jobs);
   plan->counts = reinterpret_cast<uint32_t*>(base + L.counts);
-  plan->offsets = reinterpret_cast<uint32_t*>(base + L.offsets);
+  plan->totals = reinterpret_cast<uint32_t*>(base + L.totals);
+  plan->starts = reinterpret_cast<uint32_t*>(base + L.starts);
   plan->digits = reinterpret_cast<int32_t*>(base + L.digits);
   plan->m_dev = reinterpret_cast<uint32_t*>(base + L.m);
-  plan->skey = reinterpret_cast<uint32_t*>(base + L.skey);
-  plan->sref = reinterpret_cast<uint32_t*>(base + L.sref);
+  plan->sorted = reinterpret_cast<uint2*>(base + L.sorted);
   plan->scan_ws = reinterpret_cast<uint32_t*>(base + L.scan);
   plan->buckets = reinterpret_cast<JacRec*>(base + L.buckets);
   plan->runs[0] = reinterpret_cast<RunSummary*>(base + L.r0);
@@ -757,17 +781,23 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
     hipLaunchKernelGGL(k_sort_count, dim3((unsigned)plan->sort_blocks), dim3(256), kSortRange * 4, stream, plan->d_jobs, nj,
                        plan->digits, plan->counts);
     ZK_LAUNCH_CHECK("msm sort count");
-    if (int rc = scan_u32_ws(plan->counts, plan->offsets, plan->ncounts, plan->m_dev, plan->scan_ws, stream)) return rc;
+    hipLaunchKernelGGL(k_sort_columns, dim3(grid_for(plan->nkeys, 256)), dim3(256), 0, stream, plan->d_jobs, nj,
+                       plan->nkeys, plan->counts, plan->totals);
+    ZK_LAUNCH_CHECK("msm sort columns");
+    if (int rc = scan_u32_ws(plan->totals, plan->starts, plan->nkeys, plan->m_dev, plan->scan_ws, stream)) return rc;
     hipLaunchKernelGGL(k_sort_place, dim3((unsigned)plan->sort_blocks), dim3(256), kSortRange * 4, stream, plan->d_jobs, nj,
-                       plan->digits, plan->offsets, plan->skey, plan->sref);
+                       plan->digits, plan->counts, plan->starts, plan->sorted);
     ZK_LAUNCH_CHECK("msm sort place");
   }
+  const int levels = (int)plan->level_max.size();
   {
     Prof prof("msm_accumulate", stream);
-    const int levels = (int)plan->level_max.size();
-    hipLaunchKernelGGL(k_accumulate, dim3((unsigned)plan->level_max[0]), dim3(kAccThreads), 0, stream, plan->skey,
-                       plan->sref, plan->m_dev, plan->points, plan->buckets, plan->runs[0], levels == 1 ? 1 : 0);
+    hipLaunchKernelGGL(k_accumulate, dim3((unsigned)plan->level_max[0]), dim3(kAccThreads), 0, stream, plan->sorted,
+                       plan->m_dev, plan->points, plan->buckets, plan->runs[0], levels == 1 ? 1 : 0);
     ZK_LAUNCH_CHECK("msm accumulate");
+  }
+  {
+    Prof prof("msm_combine", stream);
     int cur = 0;
     for (int l = 1; l < levels; ++l) {
       hipLaunchKernelGGL(k_combine, dim3((unsigned)plan->level_max[(size_t)l]), dim3(kCombineThreads), 0, stream,
@@ -788,12 +818,12 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
     hipLaunchKernelGGL(k_row_sums, dim3(grid_for(plan->sums_rows, 64)), dim3(64), 0, stream, plan->d_jobs, nj,
                        plan->sums_rows, plan->parts_buf, plan->rows_buf);
     ZK_LAUNCH_CHECK("msm row sums");
-    hipLaunchKernelGGL(k_window_sums, dim3(grid_for(plan->windows_total, 64)), dim3(64), 0, stream, plan->d_jobs, nj,
+    hipLaunchKernelGGL(k_window_sums, dim3((unsigned)plan->windows_total), dim3(64), 0, stream, plan->d_jobs, nj,
                        plan->windows_total, plan->rows_buf, plan->wsum);
     ZK_LAUNCH_CHECK("msm window sums");
   }
   Prof prof("msm_final", stream);
-  hipLaunchKernelGGL(k_final, dim3(grid_for(nj, 64)), dim3(64), 0, stream, plan->d_jobs, nj, plan->wsum);
+  hipLaunchKernelGGL(k_final, dim3((unsigned)nj), dim3(64), 0, stream, plan->d_jobs, nj, plan->wsum);
   ZK_LAUNCH_CHECK("msm final");
   return 0;
 }

@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -143,6 +144,12 @@ Prof::~Prof() {
 static int g_cus = 0;
 int device_cus() { return g_cus > 0 ? g_cus : 256; }
 
+// Plain grid-stride copy of 16-byte words: every byte read once and written once.
+__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t words) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) dst[i] = src[i];
+}
+
 __global__ void probe_kernel(int64_t* x, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = x[i] + 1;
@@ -249,6 +256,17 @@ int zk_memcpy_d2h(void* dst, const void* src, uint64_t bytes, void* stream) {
 int zk_memcpy_d2d(void* dst, const void* src, uint64_t bytes, void* stream) {
   if (!bytes) return 0;
   ZK_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return 0;
+}
+
+int zk_copy(void* dst, const void* src, uint64_t bytes, void* stream) {
+  if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) return fail("zk_copy needs 16-byte aligned buffers and sizes");
+  if (!bytes) return 0;
+  const int64_t words = (int64_t)(bytes / 16);
+  const int64_t blocks = std::min<int64_t>((words + 255) / 256, (int64_t)device_cus() * 16);
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, static_cast<const uint4*>(src),
+                     static_cast<uint4*>(dst), words);
+  ZK_LAUNCH_CHECK("copy");
   return 0;
 }
 

@@ -67,6 +67,7 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_memcpy_d2h": [_vp, _vp, _u64, _vp],
     "zk_memcpy_d2d": [_vp, _vp, _u64, _vp],
     "zk_memset": [_vp, _i32, _u64, _vp],
+    "zk_copy": [_vp, _vp, _u64, _vp],
     "zk_stream_create": [ctypes.POINTER(_vp)],
     "zk_stream_destroy": [_vp],
     "zk_stream_sync": [_vp],
