@@ -166,6 +166,11 @@ int zk_msm_g1(const uint64_t* points_affine, const uint64_t* scalars, int64_t n,
               int chunk_len, uint64_t* out_jac, void* stream);
 /* The signed-digit window zk_msm_g1 picks for n points when window = 0. */
 int zk_msm_default_window(int64_t n, int* window);
+/* Whether plans created afterwards split each scalar k = k1 + k2 lambda (|k1|, |k2| < 2^127)
+ * over P_i and phi(P_i) = (beta x_i, y_i) (on by default; off gives plain 254-bit digits).
+ * No counterpart in the reference, whose Pippenger recodes the full scalar (msm.py:112-207). */
+int zk_msm_set_glv(int on);
+int zk_msm_get_glv(int* on);
 /* A device-resident basis: n affine mont points converted once into the kernels' internal
  * layout and kept in HBM (the shared SRS of batched and served MSMs; no counterpart in the
  * reference, which converts its points on every call, msm_gpu.py:348). */

@@ -156,3 +156,29 @@ def test_msm_2_20_closed_form():
     scalars = workloads.witness_scalars(n, 0x5EED04)
     got = msm_g1(pts, gpu_backend.asarray(ints_to_limbs(scalars)))
     assert got.affine() == (G * workloads.closed_form_scalar(keys, scalars)).affine()
+
+
+@pytest.mark.parametrize("glv", [1, 0])
+def test_glv_split_on_and_off(glv, large_inputs):
+    """The same MSMs with the GLV split and with plain 254-bit digits: scalars that sit on the
+    split's edges (lambda, r - lambda, r - 1, >= r, 2^127 +- 1, small values) and random ones."""
+    import ctypes
+
+    lam = 0xB3C4D79D41A917585BFC41088D8DAAA78B17EA66B99C90DD
+    points, _, _ = large_inputs
+    rng = random.Random(41)
+    edge = [lam, R - lam, R - 1, R, R + 7, 2**256 - 1, 2**127, 2**127 - 1, 2**127 + 1, 1, 0, 2**32 - 1, lam + 1]
+    scalars = edge + [rng.randrange(R) for _ in range(len(points) - len(edge))]
+    expected = msm_double_and_add(points, [s % R for s in scalars])
+    old = ctypes.c_int()
+    gpu_backend.call("zk_msm_get_glv", ctypes.byref(old))
+    gpu_backend.call("zk_msm_set_glv", glv)
+    try:
+        for w in (None, 4, 9, 13):
+            assert pippenger(points, scalars, w) == aff(expected), (glv, w)
+        # all scalars on one point: every digit stream lands in a handful of buckets
+        same = [points[0]] * 64
+        s2 = [rng.randrange(R) for _ in same]
+        assert pippenger(same, s2) == aff(points[0] * (sum(s2) % R))
+    finally:
+        gpu_backend.call("zk_msm_set_glv", old.value)

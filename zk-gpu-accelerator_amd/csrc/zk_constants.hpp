@@ -38,4 +38,20 @@ struct FqIlp : FqConst {
   static constexpr bool kIlp = true;
 };
 
+// GLV split of a scalar k (< r) into k1 + k2 lambda with |k1|, |k2| < 2^127:
+// c1 = floor(k g1 / 2^256), c2 = floor(k g2 / 2^256), k1 = k - c1 a1 - c2 a2,
+// k2 = c1 b1n - c2 b2, over the lattice basis (a1, -b1n), (a2, b2) of
+// {(a, b): a + b lambda = 0 mod r}. (beta x, y) = lambda (x, y) on G1.
+// lambda = 0xb3c4d79d41a917585bfc41088d8daaa78b17ea66b99c90dd
+struct Glv {
+  static constexpr uint32_t beta261[9] = {0x0a337995u, 0x158d1d23u, 0x189c9b98u, 0x12fa4e45u, 0x185faadcu, 0x0176f16du, 0x0eed93bau, 0x14291140u, 0x000c0afeu};
+  static constexpr uint32_t a1[2] = {0x94d213e3u, 0x89d32568u};
+  static constexpr uint32_t b1n[4] = {0x7d4f1128u, 0x8211bbebu, 0xeeb859fcu, 0x6f4d8248u};
+  static constexpr uint32_t a2[4] = {0x1221250bu, 0x0be4e154u, 0xeeb859fdu, 0x6f4d8248u};
+  static constexpr uint32_t b2[2] = {0x94d213e3u, 0x89d32568u};
+  static constexpr uint32_t g1[3] = {0xc7e0b3d7u, 0xd91d232eu, 0x00000002u};
+  static constexpr uint32_t g2[5] = {0x391eb18eu, 0x7a7bd9d4u, 0xa773d2cfu, 0x4ccef014u, 0x00000002u};
+  static constexpr int kBits = 127;  // |k1|, |k2| < 2^kBits
+};
+
 }  // namespace zk

@@ -2,12 +2,18 @@
 //
 // Computes sum_i s_i * P_i with the bucket method, entirely on the device:
 //
-//  1. prep     points -> 261-form coordinates packed in 80-byte records.
-//  2. digits   every scalar (reduced mod r) is recoded into W signed digits of
-//              c bits, d in [-2^(c-1), 2^(c-1)], W = floor(254 / c) + 1; a digit
-//              d != 0 sends +-P_i to bucket (window, |d|). Signed digits halve the
-//              buckets per window against the reference's unsigned ones
-//              (src/zkgpu/msm_gpu.py:89-105).
+//  1. prep     points -> 261-form coordinates packed in 80-byte records, plus
+//              phi(P_i) = (beta x_i, y_i) = lambda P_i, the curve's endomorphism.
+//  2. digits   every scalar (reduced mod r) is split k = k1 + k2 lambda with
+//              |k1|, |k2| < 2^127 (GLV: a lattice basis of {(a, b): a + b lambda
+//              = 0 mod r}, zk_constants.hpp), and |k1| (for P_i) and |k2| (for
+//              phi(P_i)) are recoded into W = floor(127 / c) + 1 signed digits of
+//              c bits, d in [-2^(c-1), 2^(c-1)], the signs of k1, k2 folded in. A
+//              digit d != 0 sends +-P to bucket (window, |d|). Against the
+//              reference's unsigned digits of the full scalar (src/zkgpu/msm_gpu.py
+//              :89-105) this halves the buckets per window and the windows, so the
+//              bucket reduction and the final Horner chain are a quarter and a half
+//              as long; the entries grow from W254 n to 2 W127 n (about +12%).
 //  3. sort     counting sort of the (window, bucket) keys with no global atomics:
 //              digits are written window-major; one workgroup per (window, range
 //              of 16384 buckets, tile of scalars) counts its tile's digits into an
@@ -45,6 +51,7 @@
 #include "zk_curve_wave.hpp"
 #include "zk_fields.hpp"
 #include "zk_msm.hpp"
+#include "zk_scalar.hpp"
 #include "zk_scan.hpp"
 
 namespace zk {
@@ -53,42 +60,37 @@ using Fq = FqConst;
 
 // ---- packed records -------------------------------------------------------
 
-struct alignas(16) AffRec {
-  uint32_t w[20];  // x limbs 0..8, y limbs 9..17
+// A finite affine point as two 32-byte rows (the 261-form coordinates are canonical, below
+// 2^256): 64 bytes, so a gather touches half a cache line.
+struct alignas(64) AffRec {
+  uint64_t x[4], y[4];
 };
 struct alignas(16) JacRec {
-  uint32_t w[28];  // x 0..8, y 9..17, z 18..26
+  uint32_t w[28];  // x 0..8, y 9..17, z 18..26 (29-bit limbs)
 };
 
-__device__ __forceinline__ Aff load_aff_rec(const AffRec* p) {
+// The raw words of an AffRec, loaded ahead of their use.
+struct AffRaw {
+  uint4 a, b, c, d;
+};
+
+__device__ __forceinline__ AffRaw load_aff_raw(const AffRec* p) {
   const uint4* q = reinterpret_cast<const uint4*>(p);
-  uint32_t w[20];
-#pragma unroll
-  for (int i = 0; i < 5; ++i) {
-    uint4 v = q[i];
-    w[4 * i] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
-  }
+  return AffRaw{q[0], q[1], q[2], q[3]};
+}
+
+__device__ __forceinline__ Aff unpack_aff(const AffRaw& r) {
+  const uint32_t wx[8] = {r.a.x, r.a.y, r.a.z, r.a.w, r.b.x, r.b.y, r.b.z, r.b.w};
+  const uint32_t wy[8] = {r.c.x, r.c.y, r.c.z, r.c.w, r.d.x, r.d.y, r.d.z, r.d.w};
   Aff a;
-#pragma unroll
-  for (int i = 0; i < NL; ++i) {
-    a.x.l[i] = w[i];
-    a.y.l[i] = w[NL + i];
-  }
+  a.x = from_words(wx);
+  a.y = from_words(wy);
   return a;
 }
 
 __device__ __forceinline__ void store_aff_rec(AffRec* p, const Aff& a) {
-  uint32_t w[20];
-#pragma unroll
-  for (int i = 0; i < NL; ++i) {
-    w[i] = a.x.l[i];
-    w[NL + i] = a.y.l[i];
-  }
-  w[18] = 0;
-  w[19] = 0;
-  uint4* q = reinterpret_cast<uint4*>(p);
-#pragma unroll
-  for (int i = 0; i < 5; ++i) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+  store_row(p->x, a.x);
+  store_row(p->y, a.y);
 }
 
 __device__ __forceinline__ Jac load_jac_rec(const JacRec* p) {
@@ -125,56 +127,20 @@ __device__ __forceinline__ void store_jac_rec(JacRec* p, const Jac& a) {
 
 // ---- scalars and digits -----------------------------------------------------
 
-// r, little-endian 32-bit words
-__constant__ uint32_t kR32[8] = {0xf0000001u, 0x43e1f593u, 0x79b97091u, 0x2833e848u,
-                                 0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
-
-__device__ __forceinline__ void load_scalar(const uint64_t* s, uint32_t w[8]) {
-  const uint4* q = reinterpret_cast<const uint4*>(s);
-  uint4 a = q[0], b = q[1];
-  w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w; w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
-  // reduce mod r: any 256-bit value is below 6r
-  for (int rep = 0; rep < 6; ++rep) {
-    uint32_t d[8], borrow = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      uint64_t t = (uint64_t)w[i] - kR32[i] - borrow;
-      d[i] = (uint32_t)t;
-      borrow = (uint32_t)(t >> 63);
-    }
-    if (borrow) break;  // w < r
-#pragma unroll
-    for (int i = 0; i < 8; ++i) w[i] = d[i];
-  }
-}
-
-__device__ __forceinline__ uint32_t raw_window(const uint32_t w[8], int bit, int c) {
-  const int word = bit >> 5, sh = bit & 31;
-  if (word >= 8) return 0;
-  uint64_t v = w[word];
-  if (word + 1 < 8) v |= (uint64_t)w[word + 1] << 32;
-  return (uint32_t)(v >> sh) & ((1u << c) - 1u);
-}
-
-// Signed digit of window `win` given the carry in; updates the carry.
-__device__ __forceinline__ int signed_digit(const uint32_t w[8], int win, int c, uint32_t& carry) {
-  uint32_t raw = raw_window(w, win * c, c) + carry;
-  const uint32_t half = 1u << (c - 1);
-  if (raw > half) {
-    carry = 1;
-    return (int)raw - (int)(1u << c);
-  }
-  carry = 0;
-  return (int)raw;
-}
-
 // ---- kernels ----------------------------------------------------------------
 
-__global__ void __launch_bounds__(256) k_prep_points(const uint64_t* __restrict__ pts, int64_t n, AffRec* __restrict__ out) {
+// Points (256-form rows) -> 261-form records; with phi, also phi(P_i) = (beta x_i, y_i) at phi + i.
+__global__ void __launch_bounds__(256) k_prep_points(const uint64_t* __restrict__ pts, int64_t n, AffRec* __restrict__ out,
+                                                     AffRec* __restrict__ phi) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Fq f;
-  store_aff_rec(out + i, load_aff(f, pts + 8 * i));
+  Aff a = load_aff(f, pts + 8 * i);
+  store_aff_rec(out + i, a);
+  if (phi) {
+    a.x = mont(f, a.x, fe_const<Fq>(Glv::beta261));
+    store_aff_rec(phi + i, a);
+  }
 }
 
 // Per-MSM parameters of a batch (see MsmPlan).
@@ -192,7 +158,9 @@ struct MsmJob {
   int64_t cnt_base;         // first (bucket, tile) count
   int64_t tile;             // scalars per sort tile
   uint64_t* out;            // 12 limbs, the Jacobian result
-  int c, windows, log_seg, log_segs, chunks, tiles, ranges;
+  int64_t nv;               // digit streams: n, or 2n with the GLV split (k1 for P_i, k2 for phi(P_i))
+  int64_t phi_base;         // record index of phi(P_0) in the point array
+  int c, windows, log_seg, log_segs, chunks, tiles, ranges, glv;
 };
 
 // Largest j with base(j) <= x for a field of the job table (bases increase).
@@ -207,16 +175,36 @@ __device__ __forceinline__ int find_job(const MsmJob* jobs, int njobs, int64_t x
   return lo;
 }
 
-// Signed digits of every scalar: digits[dig_base + w * n + i] = d (0 for none).
+// Signed digits of every scalar: digits[dig_base + w * nv + v] = d (0 for none), v = i for the
+// scalar itself, or with the GLV split v = i for k1 (point P_i) and v = n + i for k2 (phi(P_i)),
+// the sign of k1 / k2 folded into the digits.
 __global__ void __launch_bounds__(256) k_signed_digits(const MsmJob* __restrict__ jobs, int njobs,
                                                        int32_t* __restrict__ digits) {
   const MsmJob& job = jobs[find_job<&MsmJob::block_base>(jobs, njobs, blockIdx.x)];
   const int64_t i = ((int64_t)blockIdx.x - job.block_base) * 256 + threadIdx.x;
   if (i >= job.n) return;
   uint32_t w[8];
-  load_scalar(job.scalars + 4 * i, w);
-  uint32_t carry = 0;
-  for (int win = 0; win < job.windows; ++win) digits[job.dig_base + (int64_t)win * job.n + i] = signed_digit(w, win, job.c, carry);
+  {
+    const uint4* q = reinterpret_cast<const uint4*>(job.scalars + 4 * i);
+    const uint4 a = q[0], b = q[1];
+    w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w; w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+  }
+  reduce_mod_r(w);
+  int32_t* d = digits + job.dig_base;
+  if (!job.glv) {
+    uint32_t carry = 0;
+    for (int win = 0; win < job.windows; ++win) d[(int64_t)win * job.nv + i] = signed_digit(w, win, job.c, carry);
+    return;
+  }
+  uint32_t m1[8], m2[8];
+  bool neg1, neg2;
+  glv_split(w, m1, neg1, m2, neg2);
+  uint32_t c1 = 0, c2 = 0;
+  for (int win = 0; win < job.windows; ++win) {
+    const int d1 = signed_digit(m1, win, job.c, c1), d2 = signed_digit(m2, win, job.c, c2);
+    d[(int64_t)win * job.nv + i] = neg1 ? -d1 : d1;
+    d[(int64_t)win * job.nv + job.n + i] = neg2 ? -d2 : d2;
+  }
 }
 
 constexpr int kSortRange = 16384;  // buckets per sort block (its LDS holds one counter per bucket)
@@ -252,8 +240,8 @@ __global__ void __launch_bounds__(256) k_sort_count(const MsmJob* __restrict__ j
   const int range = (int)min((int64_t)kSortRange, buckets - r0);
   for (int k = threadIdx.x; k < range; k += 256) hist[k] = 0;
   __syncthreads();
-  const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.n, lo + job.tile);
-  const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.n;
+  const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.nv, lo + job.tile);
+  const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.nv;
   for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
     const int32_t v = d[i];
     const int64_t b = (int64_t)(v < 0 ? -v : v) - 1 - r0;  // bucket index within the range
@@ -300,14 +288,15 @@ __global__ void __launch_bounds__(256) k_sort_place(const MsmJob* __restrict__ j
   const int64_t key0 = job.key_base + (int64_t)sb.win * buckets + r0;
   for (int k = threadIdx.x; k < range; k += 256) cur[k] = starts[key0 + k] + off[k];
   __syncthreads();
-  const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.n, lo + job.tile);
-  const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.n;
+  const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.nv, lo + job.tile);
+  const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.nv;
   for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
     const int32_t v = d[i];
     const int64_t b = (int64_t)(v < 0 ? -v : v) - 1 - r0;
     if (v != 0 && b >= 0 && b < range) {
       const uint32_t pos = atomicAdd(&cur[b], 1u);
-      sorted[pos] = make_uint2((uint32_t)(key0 + b), (uint32_t)i | (v < 0 ? 0x80000000u : 0u));
+      const int64_t rec = i < job.n ? i : job.phi_base + (i - job.n);  // the stream's point record
+      sorted[pos] = make_uint2((uint32_t)(key0 + b), (uint32_t)rec | (v < 0 ? 0x80000000u : 0u));
     }
   }
 }
@@ -407,15 +396,21 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
   me.flags = 0;
   me.fk = me.lk = 0;
   if (s < m) {
-    const int64_t e = min(s + k, m);
+    const int64_t e = min(s + k, m), last = e - 1;
     Jac acc = jac_identity(f);
-    uint32_t cur = sorted[s].x;
+    // software pipeline: the point of entry x and the record of entry x + 1 are in flight while
+    // entry x - 1 is added (indices past the range are clamped to its last entry, and unused)
+    uint2 rec = sorted[s];
+    uint2 rec1 = sorted[min(s + 1, last)];
+    AffRaw pt = load_aff_raw(pts + (rec.y & 0x7fffffffu));
+    uint32_t cur = rec.x;
     bool first = true;
     me.fk = cur;
     me.flags = kValid | kSingle;
     for (int64_t x = s; x <= e; ++x) {
-      const uint2 rec = x < e ? sorted[x] : make_uint2(0xffffffffu, 0u);
-      const uint32_t key = rec.x;
+      const uint2 rec2 = sorted[min(x + 2, last)];
+      const AffRaw pt1 = load_aff_raw(pts + (rec1.y & 0x7fffffffu));
+      const uint32_t key = x < e ? rec.x : 0xffffffffu;
       if (key != cur) {
         const bool is_last = x == e;
         if (first && is_last) {
@@ -434,10 +429,12 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
         cur = key;
         acc = jac_identity(f);
       }
-      const uint32_t ref = rec.y;
-      Aff q = load_aff_rec(pts + (ref & 0x7fffffffu));
-      if (ref >> 31) q.y = fneg(f, q.y);
+      Aff q = unpack_aff(pt);
+      if (rec.y >> 31) q.y = fneg(f, q.y);
       acc = jac_madd(f, acc, q);
+      rec = rec1;
+      rec1 = rec2;
+      pt = pt1;
     }
   }
   sh[threadIdx.x] = me;
@@ -575,13 +572,19 @@ __global__ void __launch_bounds__(64) k_final(const MsmJob* __restrict__ jobs, i
 
 // ---- host ---------------------------------------------------------------------
 
+// Whether new plans split scalars with the GLV endomorphism (zk_msm_set_glv).
+static int g_glv = 1;
+
 int default_window(int64_t n) {
-  // Signed digits: about n additions per window plus ~3 * 2^(c-1) for the reduction.
+  // Signed digits: about n additions per window (2n over half as many windows with the GLV
+  // split) plus ~3 * 2^(c-1) for each window's bucket reduction.
+  const int bits = g_glv ? Glv::kBits : 254;
+  const double streams = g_glv ? 2.0 : 1.0;
   int best = 1;
   double best_cost = 1e300;
   for (int c = 1; c <= 20; ++c) {
-    const int windows = 254 / c + 1;
-    double cost = (double)windows * ((double)n + 3.0 * (double)(1 << (c - 1)));
+    const int windows = bits / c + 1;
+    double cost = (double)windows * (streams * (double)n + 3.0 * (double)(1 << (c - 1)));
     if (cost < best_cost) {
       best_cost = cost;
       best = c;
@@ -624,10 +627,12 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   if (chunk < 0) return fail("chunk_len must be positive (or 0 for automatic)");
   int64_t blocks = 0, keys = 0, entries = 0, segs = 0, sums = 0, parts = 0, wins = 0, sort_blocks = 0, cnts = 0;
   for (auto& j : jobs) {
-    if (j.n < 0 || j.n > 0x7fffffff) return fail("MSM size must be in [0, 2^31)");
+    if (j.n < 0 || j.n > 0x3fffffff) return fail("MSM size must be in [0, 2^30)");
     if (j.c <= 0) j.c = default_window(j.n);
     if (j.c > 20) return fail("window must be in 1..20");
-    j.windows = 254 / j.c + 1;
+    j.glv = g_glv;
+    j.nv = j.glv ? 2 * j.n : j.n;
+    j.windows = (j.glv ? Glv::kBits : 254) / j.c + 1;
     j.log_seg = std::min(2, j.c - 1);
     j.log_segs = (j.c - 1) - j.log_seg;
     j.chunks = (int)(((1LL << j.log_segs) + kRedChunk - 1) / kRedChunk);
@@ -641,8 +646,8 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
     const int64_t nb = 1LL << (j.c - 1);
     j.ranges = (int)((nb + kSortRange - 1) / kSortRange);
     const int64_t want = (1024 + (int64_t)j.windows * j.ranges - 1) / ((int64_t)j.windows * j.ranges);
-    j.tiles = (int)std::max<int64_t>(1, std::min<int64_t>(want, (j.n + 2047) / 2048));
-    j.tile = std::max<int64_t>(1, (j.n + j.tiles - 1) / j.tiles);
+    j.tiles = (int)std::max<int64_t>(1, std::min<int64_t>(want, (j.nv + 2047) / 2048));
+    j.tile = std::max<int64_t>(1, (j.nv + j.tiles - 1) / j.tiles);
     j.dig_base = entries;
     j.sort_base = sort_blocks;
     j.cnt_base = cnts;
@@ -650,7 +655,7 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
     cnts += (int64_t)j.windows * nb * j.tiles;
     blocks += std::max<int64_t>(1, (j.n + 255) / 256);
     keys += (int64_t)j.windows << (j.c - 1);
-    entries += (int64_t)j.windows * j.n;
+    entries += (int64_t)j.windows * j.nv;
     segs += (int64_t)j.windows << j.log_segs;
     sums += (int64_t)j.windows * (j.log_segs + 1);
     parts += (int64_t)j.windows * (j.log_segs + 1) * j.chunks;
@@ -828,27 +833,31 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
   return 0;
 }
 
-// Points (n, 2, 4) 256-form rows -> packed 261-form records.
-int convert_points(const uint64_t* points, int64_t n, AffRec* out, hipStream_t stream) {
+// Points (n, 2, 4) 256-form rows -> packed 261-form records at out[0, n), and phi(P_i) at
+// out[n, 2n) when with_phi.
+int convert_points(const uint64_t* points, int64_t n, AffRec* out, bool with_phi, hipStream_t stream) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_prep_points, dim3(grid_for(n, 256)), dim3(256), 0, stream, points, n, out);
+  hipLaunchKernelGGL(k_prep_points, dim3(grid_for(n, 256)), dim3(256), 0, stream, points, n, out,
+                     with_phi ? out + n : nullptr);
   ZK_LAUNCH_CHECK("msm prep");
   return 0;
 }
 
 int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, int k0, uint64_t* out_jac,
            hipStream_t stream) {
-  if (n < 0 || n > 0x7fffffff) return fail("MSM size must be in [0, 2^31)");
+  if (n < 0 || n > 0x3fffffff) return fail("MSM size must be in [0, 2^30)");
   if (c < 0 || c > 20) return fail("window must be in 1..20");
+  const bool glv = g_glv != 0;
   Scratch sc(stream);
-  AffRec* pts = sc.get<AffRec>((size_t)std::max<int64_t>(n, 1));
+  AffRec* pts = sc.get<AffRec>((size_t)std::max<int64_t>(glv ? 2 * n : n, 1));
   if (!pts) return 1;
-  if (int rc = convert_points(points, n, pts, stream)) return rc;
+  if (int rc = convert_points(points, n, pts, glv, stream)) return rc;
   MsmJob job{};
   job.scalars = scalars;
   job.n = n;
   job.c = c;
   job.out = out_jac;
+  job.phi_base = n;
   MsmPlan* plan = nullptr;
   if (int rc = msm_plan_create(pts, {job}, k0, stream, &plan)) return rc;
   int rc = msm_plan_launch(plan, stream);
@@ -867,6 +876,16 @@ int zk_msm_default_window(int64_t n, int* window) {
   return 0;
 }
 
+int zk_msm_set_glv(int on) {
+  g_glv = on ? 1 : 0;
+  return 0;
+}
+
+int zk_msm_get_glv(int* on) {
+  *on = g_glv;
+  return 0;
+}
+
 int zk_msm_g1(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window, int chunk_len,
               uint64_t* out_jac, void* stream) {
   return msm_g1(points_affine, scalars, n, window, chunk_len, out_jac, (hipStream_t)stream);
@@ -878,10 +897,10 @@ struct ZkSrs {
 };
 
 int zk_srs_create(const uint64_t* points_affine, int64_t n, void** srs, void* stream) {
-  if (n <= 0 || n > 0x7fffffff) return fail("SRS size must be in [1, 2^31)");
+  if (n <= 0 || n > 0x3fffffff) return fail("SRS size must be in [1, 2^30)");
   AffRec* p = nullptr;
-  ZK_HIP(hipMalloc(&p, sizeof(AffRec) * (size_t)n));
-  if (int rc = convert_points(points_affine, n, p, (hipStream_t)stream)) {
+  ZK_HIP(hipMalloc(&p, sizeof(AffRec) * 2 * (size_t)n));  // P_i, then phi(P_i)
+  if (int rc = convert_points(points_affine, n, p, true, (hipStream_t)stream)) {
     (void)hipFree(p);
     return rc;
   }
@@ -908,6 +927,7 @@ int zk_msm_plan_create(void* srs, int count, const uint64_t* const* scalars, con
     if (n[j] > s->n) return fail("MSM size exceeds the SRS");
     jobs[(size_t)j] = MsmJob{};
     jobs[(size_t)j].scalars = scalars[j];
+    jobs[(size_t)j].phi_base = s->n;
     jobs[(size_t)j].n = n[j];
     jobs[(size_t)j].c = window ? window[j] : 0;
     jobs[(size_t)j].out = out_jac + 12 * j;

@@ -86,6 +86,8 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_graph_destroy": [_vp],
     "zk_valu_peak": [ctypes.POINTER(ctypes.c_double)],
     "zk_msm_default_window": [_i64, ctypes.POINTER(_i32)],
+    "zk_msm_set_glv": [_i32],
+    "zk_msm_get_glv": [ctypes.POINTER(_i32)],
     "zk_prof_enable": [_i32],
     "zk_prof_read": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), _i64p, _i32],
     "zk_field_register": [_u64p, ctypes.POINTER(_i32)],
