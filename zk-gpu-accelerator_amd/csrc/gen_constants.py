@@ -20,6 +20,7 @@ def limbs29(v: int) -> list[int]:
 def field_constants(m: int) -> dict[str, list[int] | int]:
     return {
         "p": limbs29(m),
+        "p2": limbs29(2 * m),              # 2p, the bound of lazily reduced values
         "ninv": (-pow(m, -1, 1 << LIMB_BITS)) % (1 << LIMB_BITS),
         "c256": limbs29(pow(2, 256, m)),   # multiplier taking a 2^261 form to the 2^256 form
         "c266": limbs29(pow(2, 266, m)),   # multiplier taking the 2^256 form to the 2^261 form
@@ -33,7 +34,7 @@ def field_constants(m: int) -> dict[str, list[int] | int]:
 def emit(name: str, m: int) -> str:
     c = field_constants(m)
     out = [f"struct {name} {{", f"  // modulus {m:#x}"]
-    for key in ("p", "c256", "c266", "c517", "c522", "one"):
+    for key in ("p", "p2", "c256", "c266", "c517", "c522", "one"):
         vals = ", ".join(f"0x{v:08x}u" for v in c[key])
         out.append(f"  static constexpr uint32_t {key}[9] = {{{vals}}};")
     out.append(f"  static constexpr uint32_t p32[8] = {{{', '.join(f'0x{v:08x}u' for v in c['p32'])}}};")

@@ -45,6 +45,7 @@ struct Fe {
 // read f.p[i] whether the field is built in or registered.
 struct FieldRT {
   uint32_t p[NL];
+  uint32_t p2[NL];  // 2p
   uint32_t c256[NL];
   uint32_t c266[NL];
   uint32_t c517[NL];
@@ -260,6 +261,68 @@ __device__ __forceinline__ Fe fsub(const F& f, const Fe& a, const Fe& b) {
 #pragma unroll
   for (int i = 0; i < NL; ++i) {
     uint32_t t = d.l[i] + (f.p[i] & mask) + c;
+    d.l[i] = t & LMASK;
+    c = t >> LB;
+  }
+  return d;
+}
+
+// ---- lazily reduced arithmetic (for long chains of additions and products, e.g. NTT passes) ----
+// Values are kept in [0, 2p). mont_raw of operands below 4p and p stays below 2p (4p * p
+// < p * 2^261), so a product needs no final subtraction.
+
+// (a + b) reduced into [0, 2p), for a, b < 2p.
+template <class F>
+__device__ __forceinline__ Fe fadd_lazy(const F& f, const Fe& a, const Fe& b) {
+  Fe s, d;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    uint32_t t = a.l[i] + b.l[i] + c;
+    s.l[i] = t & LMASK;
+    c = t >> LB;
+  }
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    uint32_t t = s.l[i] - f.p2[i] - borrow;
+    borrow = t >> 31;
+    d.l[i] = t & LMASK;
+  }
+#pragma unroll
+  for (int i = 0; i < NL; ++i) s.l[i] = borrow ? s.l[i] : d.l[i];
+  return s;
+}
+
+// a - b + 2p, in (0, 4p) for a, b < 2p, unreduced: the operand of a product.
+template <class F>
+__device__ __forceinline__ Fe fsub_for_mul(const F& f, const Fe& a, const Fe& b) {
+  Fe d;
+  int32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const int32_t t = (int32_t)(a.l[i] + f.p2[i]) - (int32_t)b.l[i] + c;  // |t| < 2^31
+    d.l[i] = (uint32_t)t & LMASK;
+    c = t >> LB;  // arithmetic: the signed carry
+  }
+  return d;
+}
+
+// a - b in [0, 2p) for a, b < 2p.
+template <class F>
+__device__ __forceinline__ Fe fsub_lazy(const F& f, const Fe& a, const Fe& b) {
+  Fe d;
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    uint32_t t = a.l[i] - b.l[i] - borrow;
+    borrow = t >> 31;
+    d.l[i] = t & LMASK;
+  }
+  uint32_t mask = 0u - borrow, c = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    uint32_t t = d.l[i] + (f.p2[i] & mask) + c;
     d.l[i] = t & LMASK;
     c = t >> LB;
   }

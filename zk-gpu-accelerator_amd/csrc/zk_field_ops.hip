@@ -80,6 +80,13 @@ bool make_field_rt(const uint64_t modulus[4], FieldRT* out, std::string* err) {
     return false;
   }
   to_limbs29(m, out->p);
+  Big m2;  // 2p < 2^256
+  uint64_t carry = 0;
+  for (int i = 0; i < 4; ++i) {
+    m2.w[i] = (m.w[i] << 1) | carry;
+    carry = m.w[i] >> 63;
+  }
+  to_limbs29(m2, out->p2);
   to_limbs29(pow2_mod(256, m), out->c256);
   to_limbs29(pow2_mod(266, m), out->c266);
   to_limbs29(pow2_mod(517, m), out->c517);

@@ -18,6 +18,12 @@
 // reversal its writes are again C consecutive rows. 2^22 runs as 8 + 7 + 7
 // stages: three read+write sweeps of HBM instead of the reference's 23.
 //
+// Reduction. Inside a pass values stay in [0, 2p): a sum is brought back under
+// 2p, a difference u - v + 2p goes straight into the product, whose result is
+// below 2p with no final subtraction; intermediate passes store such values,
+// the last pass stores canonical ones. The last stage of a transform has the
+// single twiddle 1 and does no product.
+//
 // Twiddles. One table per stage, w_s[j] = omega^(j * 2^s), so that a pass
 // reads each stage's twiddles contiguously; values are in the 261-form, so
 // (u - v) * w keeps the data's 256-form without conversions. Coset powers and
@@ -118,6 +124,7 @@ __device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64
     const int hb = geo.k - 1 - t;  // bit of m that this stage pairs
     const int s = geo.s0 + t;
     const uint64_t* tws = d.tw + 4 * (n - (n >> s));  // stage s table: n >> (s + 1) entries
+    const bool last_stage = s == geo.log_n - 1;        // its only twiddle is 1
     for (int b = threadIdx.x; b < (T >> 1); b += kThreads) {
       const int g = b & (C - 1), qv = b >> geo.log_c;
       const int mlow = qv & ((1 << hb) - 1);
@@ -125,11 +132,15 @@ __device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64
       const int m1 = m0 | (1 << hb);
       const int64_t row0 = geo.row(wg, g, m0);
       const int64_t j = ((int64_t)mlow << log_s) + (row0 & ((1LL << log_s) - 1));
-      Fe w = load_row(tws + 4 * j);
       const int s0slot = (m0 << geo.log_c) + g, s1slot = (m1 << geo.log_c) + g;
       Fe u = lds_get(lds, s0slot), v = lds_get(lds, s1slot);
-      lds_put(lds, s0slot, fadd(f, u, v));
-      lds_put(lds, s1slot, mont(f, fsub(f, u, v), w));
+      lds_put(lds, s0slot, fadd_lazy(f, u, v));
+      if (last_stage) {
+        lds_put(lds, s1slot, fsub_lazy(f, u, v));  // w = omega^0 = 1
+      } else {
+        Fe w = load_row(tws + 4 * j);
+        lds_put(lds, s1slot, mont_raw(f, fsub_for_mul(f, u, v), w));
+      }
     }
     __syncthreads();
   }
@@ -137,9 +148,10 @@ __device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64
   for (int e = threadIdx.x; e < T; e += kThreads) {
     const int g = e & (C - 1), m = e >> geo.log_c;
     const int64_t i = geo.row(wg, g, m);
-    Fe x = lds_get(lds, e);
+    Fe x = lds_get(lds, e);  // below 2p: passes hand each other lazily reduced rows
     const int64_t o = geo.last ? bitrev(i, geo.log_n) : i;
     if (d.post) x = mont(f, x, load_row(d.post + (d.post_is_table ? 4 * o : 0)));
+    else if (geo.last) x = canonical(f, x);
     store_row(d.dst + 4 * o, x);
   }
 }
