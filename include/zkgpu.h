@@ -171,11 +171,23 @@ int zk_msm_default_window(int64_t n, int* window);
  * No counterpart in the reference, whose Pippenger recodes the full scalar (msm.py:112-207). */
 int zk_msm_set_glv(int on);
 int zk_msm_get_glv(int* on);
+/* The bucket accumulation kernel plans created afterwards use (picked by the autotuner):
+ * bit 0 = the low-latency product schedule, bit 1 = XYZZ bucket coordinates (madd-2008-s);
+ * 0 = Jacobian madd-2007-bl with serial column products. No counterpart in the reference. */
+int zk_msm_set_accumulate(int variant);
+int zk_msm_get_accumulate(int* variant);
 /* A device-resident basis: n affine mont points converted once into the kernels' internal
  * layout and kept in HBM (the shared SRS of batched and served MSMs; no counterpart in the
  * reference, which converts its points on every call, msm_gpu.py:348). */
 int zk_srs_create(const uint64_t* points_affine, int64_t n, void** srs, void* stream);
 int zk_srs_destroy(void* srs);
+/* Adds a window table of width c (4..20) to an SRS: 2^(c w) P_i for every window w of a 254-bit
+ * scalar, W = floor(254 / c) + 1 windows, W x 64 bytes per point of HBM. MSM plans over the SRS
+ * then send every window's digits to one shared set of 2^(c-1) buckets (one bucket reduction,
+ * no window combination), each job using the table with the least work for its size, or the
+ * table of its requested window. Fails while plans over the SRS exist. No counterpart in the
+ * reference, which has no resident basis. */
+int zk_srs_precompute(void* srs, int window, void* stream);
 /* A batch of count MSMs over prefixes of one SRS: job j sums n[j] <= SRS size points with the
  * plain scalars at scalars[j] (device address, n[j] rows) and writes its Jacobian mont result to
  * out_jac + 12 * j. window[j] may be 0 (automatic) or window may be NULL; chunk_len 0 = automatic.

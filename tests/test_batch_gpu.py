@@ -99,3 +99,42 @@ def test_graph_replays_match_eager(srs):
         assert [aff(p) for p in msm.results()] == want_msm, rep
         for o, w in zip(outs, want_ntt):
             assert np.array_equal(BN254_FR.to_limbs(o).get(), w), rep
+
+
+@pytest.fixture(scope="module")
+def tabled():
+    keys = workloads.srs_keys(1 << 13, 0x5EED09)
+    basis = SRS.from_keys(ints_to_limbs(keys)).precompute([8, 12])
+    return keys, basis
+
+
+def test_window_tables_match_closed_forms(tabled):
+    """MSMs over an SRS with window tables (every window's digits in one set of buckets)."""
+    keys, basis = tabled
+    sizes = [1, 2, 7, 100, 1000, 4096, 8192, 3000]
+    scalars = [workloads.witness_scalars(n, 2000 + j) for j, n in enumerate(sizes)]
+    for windows in (None, [8, 12, 8, 12, 12, 8, 12, 0]):
+        batch = MsmBatch(basis, [gb.asarray(ints_to_limbs(s)) for s in scalars], windows=windows)
+        batch.launch()
+        assert [aff(p) for p in batch.results()] == [expected(keys[:n], s) for n, s in zip(sizes, scalars)]
+
+
+def test_window_tables_edge_scalars_and_mixed_jobs(tabled):
+    keys, basis = tabled
+    cases = [[0, 0, 0], [1], [R - 1, 1], [2**256 - 1, R, R + 5], [R - 1] * 500]
+    # window 9 has no table: that job uses its own windows beside the table jobs
+    batch = MsmBatch(basis, [gb.asarray(ints_to_limbs(c)) for c in cases], windows=[0, 12, 9, 8, 0], chunk_len=2)
+    batch.launch()
+    got = [aff(p) for p in batch.results()]
+    assert got == [expected(keys[: len(c)], [x % R for x in c]) for c in cases]
+
+
+def test_window_table_needs_no_live_plan(tabled):
+    _, basis = tabled
+    plan = MsmBatch(basis, [gb.asarray(ints_to_limbs([1, 2]))])
+    with pytest.raises(gb.ZkError, match="before plans"):
+        basis.precompute([16])
+    del plan
+    import gc
+
+    gc.collect()

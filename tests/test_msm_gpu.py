@@ -182,3 +182,27 @@ def test_glv_split_on_and_off(glv, large_inputs):
         assert pippenger(same, s2) == aff(points[0] * (sum(s2) % R))
     finally:
         gpu_backend.call("zk_msm_set_glv", old.value)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_accumulation_variants(variant):
+    """Every bucket accumulation kernel (Jacobian / XYZZ, serial / low-latency products)."""
+    import ctypes
+
+    n = 3000
+    keys = workloads.srs_keys(n, 0x5EED0C)
+    scalars = workloads.witness_scalars(n, 0x5EED0D) + []
+    scalars[:4] = [R - 1, 1, 1, 0]  # repeated small digits: long runs in few buckets
+    pts = workloads.srs_points(ints_to_limbs(keys))
+    old = ctypes.c_int()
+    gpu_backend.call("zk_msm_get_accumulate", ctypes.byref(old))
+    gpu_backend.call("zk_msm_set_accumulate", variant)
+    try:
+        e = workloads.closed_form_scalar(keys, scalars)
+        for chunk in (0, 2, 7):
+            out = gpu_backend.empty((3, 4))
+            msm_gpu.msm_g1_jacobian(pts, gpu_backend.asarray(ints_to_limbs(scalars)), out, chunk_len=chunk)
+            got = msm_gpu._result(out)
+            assert (got.is_identity() and e == 0) or got.affine() == (G * e).affine(), (variant, chunk)
+    finally:
+        gpu_backend.call("zk_msm_set_accumulate", old.value)

@@ -125,3 +125,14 @@ def test_budget_limits_inflight_work(served):
     rep = server.serve(jobs)
     assert rep.peak_bytes <= 2 * one
     check_against_alone(jobs[:4], srs, pools)
+
+
+def test_served_over_window_tables_equal_alone():
+    """A server over an SRS with window tables; each job checked against its per-call run."""
+    keys = workloads.srs_keys(1 << 13, 0x5EED0A)
+    srs = SRS.from_keys(ints_to_limbs(keys)).precompute([12])
+    pools = WitnessPools((1 << 13) + 77, seed=0x5EED0B)
+    jobs = served_workload(pools, 10, None, lo=8, hi=13, seed=14)
+    rep = ProofServer(srs, hbm_budget=1 << 31, slots=2, max_batch_jobs=4).serve(jobs)
+    assert rep.batches >= 3
+    check_against_alone(jobs, srs, pools)

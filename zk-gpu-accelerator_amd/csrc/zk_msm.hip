@@ -25,9 +25,10 @@
 //              all entries into a few buckets (booleans send ~15% of them to
 //              bucket 1 of window 0); those hot keys only ever meet LDS atomics.
 //  4. buckets  a persistent grid splits the m sorted entries evenly, ceil(m / threads)
-//              consecutive entries per thread, added with mixed Jacobian + affine
-//              additions; a run of equal keys strictly inside a thread's range is a
-//              whole bucket and is written at once. The runs cut by range
+//              consecutive entries per thread, added to an XYZZ accumulator with
+//              affine points (madd-2008-s: 10 products instead of Jacobian's 11);
+//              a run of equal keys strictly inside a thread's range is a whole
+//              bucket and is written at once, as a Jacobian point. The runs cut by range
 //              boundaries are merged by a tree of run summaries (workgroup trees in
 //              LDS, then levels of 256), each merge at most one addition. The load
 //              is balanced whatever the bucket sizes, with no host sync (the
@@ -49,6 +50,7 @@
 #include "../../include/zkgpu.h"
 #include "zk_curve.hpp"
 #include "zk_curve_wave.hpp"
+#include "zk_curve_xyzz.hpp"
 #include "zk_fields.hpp"
 #include "zk_msm.hpp"
 #include "zk_scalar.hpp"
@@ -160,7 +162,11 @@ struct MsmJob {
   uint64_t* out;            // 12 limbs, the Jacobian result
   int64_t nv;               // digit streams: n, or 2n with the GLV split (k1 for P_i, k2 for phi(P_i))
   int64_t phi_base;         // record index of phi(P_0) in the point array
+  int64_t table_base;       // with a window table: record index of 2^(c w) P_i at table_base + w * stride + i
+  int64_t table_stride;
   int c, windows, log_seg, log_segs, chunks, tiles, ranges, glv;
+  int table;                // 1: every window's digits share one set of buckets (the SRS's window table)
+  int rw;                   // windows the bucket reduction and the final Horner chain see (1 with a table)
 };
 
 // Largest j with base(j) <= x for a field of the job table (bases increase).
@@ -263,9 +269,11 @@ __global__ void __launch_bounds__(256) k_sort_columns(const MsmJob* __restrict__
   const int64_t local = key - job.key_base;
   const int64_t buckets = 1LL << (job.c - 1);
   const int64_t win = local >> (job.c - 1), b = local & (buckets - 1);
-  uint32_t* col = counts + job.cnt_base + win * job.tiles * buckets + b;
+  // a key's rows: the tiles of its window, or of every window when they share the buckets
+  uint32_t* col = counts + job.cnt_base + (job.table ? 0 : win * job.tiles * buckets) + b;
+  const int rows = job.table ? job.windows * job.tiles : job.tiles;
   uint32_t run = 0;
-  for (int t = 0; t < job.tiles; ++t) {
+  for (int t = 0; t < rows; ++t) {
     const uint32_t v = col[(int64_t)t * buckets];
     col[(int64_t)t * buckets] = run;
     run += v;
@@ -285,7 +293,7 @@ __global__ void __launch_bounds__(256) k_sort_place(const MsmJob* __restrict__ j
   const int64_t r0 = (int64_t)sb.range * kSortRange;
   const int range = (int)min((int64_t)kSortRange, buckets - r0);
   const uint32_t* off = counts + job.cnt_base + ((int64_t)sb.win * job.tiles + sb.tile) * buckets + r0;
-  const int64_t key0 = job.key_base + (int64_t)sb.win * buckets + r0;
+  const int64_t key0 = job.key_base + (job.table ? 0 : (int64_t)sb.win * buckets) + r0;
   for (int k = threadIdx.x; k < range; k += 256) cur[k] = starts[key0 + k] + off[k];
   __syncthreads();
   const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.nv, lo + job.tile);
@@ -295,7 +303,9 @@ __global__ void __launch_bounds__(256) k_sort_place(const MsmJob* __restrict__ j
     const int64_t b = (int64_t)(v < 0 ? -v : v) - 1 - r0;
     if (v != 0 && b >= 0 && b < range) {
       const uint32_t pos = atomicAdd(&cur[b], 1u);
-      const int64_t rec = i < job.n ? i : job.phi_base + (i - job.n);  // the stream's point record
+      // the point record: 2^(c w) P_i from the window table, P_i, or phi(P_i) for the k2 stream
+      const int64_t rec = job.table ? job.table_base + (int64_t)sb.win * job.table_stride + i
+                                    : (i < job.n ? i : job.phi_base + (i - job.n));
       sorted[pos] = make_uint2((uint32_t)(key0 + b), (uint32_t)rec | (v < 0 ? 0x80000000u : 0u));
     }
   }
@@ -381,7 +391,30 @@ constexpr int kAccThreads = 64;      // one wave per accumulation workgroup: its
 constexpr int kCombineThreads = 256;
 
 // Level 0, a persistent grid: the m sorted entries are split evenly, ceil(m / threads) consecutive
-// entries per thread (mixed additions), and each workgroup merges its threads' run summaries.
+// entries per thread, added in XYZZ coordinates (madd-2008-s), and each workgroup merges its
+// threads' run summaries; sums leave as Jacobian points.
+// The running bucket sum of the accumulation, in Jacobian (madd-2007-bl) or XYZZ (madd-2008-s)
+// coordinates; F picks the product schedule (serial columns, or FqIlp's independent partial sums).
+template <class F, bool kXyzz>
+struct BucketSum;
+
+template <class F>
+struct BucketSum<F, false> {
+  Jac a;
+  __device__ __forceinline__ void reset(const F& f) { a = jac_identity(f); }
+  __device__ __forceinline__ void add(const F& f, const Aff& q) { a = jac_madd(f, a, q); }
+  __device__ __forceinline__ Jac jac(const F&) const { return a; }
+};
+
+template <class F>
+struct BucketSum<F, true> {
+  Xyzz a;
+  __device__ __forceinline__ void reset(const F& f) { a = xyzz_identity(f); }
+  __device__ __forceinline__ void add(const F& f, const Aff& q) { a = xyzz_madd(f, a, q); }
+  __device__ __forceinline__ Jac jac(const F& f) const { return xyzz_to_jac(f, a); }
+};
+
+template <class F, bool kXyzz>
 __global__ void __launch_bounds__(kAccThreads)
 k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_dev,
              const AffRec* __restrict__ pts, JacRec* __restrict__ buckets, RunSummary* __restrict__ out, int last_level) {
@@ -391,13 +424,14 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
   const int64_t m = *m_dev;
   const int64_t k = (m + threads - 1) / threads;
   const int64_t s = t * k;
-  const Fq f;
+  const F f;
   RunSummary me;
   me.flags = 0;
   me.fk = me.lk = 0;
   if (s < m) {
     const int64_t e = min(s + k, m), last = e - 1;
-    Jac acc = jac_identity(f);
+    BucketSum<F, kXyzz> acc;
+    acc.reset(f);
     // software pipeline: the point of entry x and the record of entry x + 1 are in flight while
     // entry x - 1 is added (indices past the range are clamped to its last entry, and unused)
     uint2 rec = sorted[s];
@@ -414,24 +448,24 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
       if (key != cur) {
         const bool is_last = x == e;
         if (first && is_last) {
-          store_jac_rec(&me.fsum, acc);
+          store_jac_rec(&me.fsum, acc.jac(f));
         } else if (first) {
-          store_jac_rec(&me.fsum, acc);
+          store_jac_rec(&me.fsum, acc.jac(f));
           me.flags = kValid;
         } else if (is_last) {
           me.lk = cur;
-          store_jac_rec(&me.lsum, acc);
+          store_jac_rec(&me.lsum, acc.jac(f));
         } else {
-          store_jac_rec(buckets + cur, acc);  // interior run: complete
+          store_jac_rec(buckets + cur, acc.jac(f));  // interior run: complete
         }
         if (is_last) break;
         first = false;
         cur = key;
-        acc = jac_identity(f);
+        acc.reset(f);
       }
       Aff q = unpack_aff(pt);
       if (rec.y >> 31) q.y = fneg(f, q.y);
-      acc = jac_madd(f, acc, q);
+      acc.add(f, q);
       rec = rec1;
       rec1 = rec2;
       pt = pt1;
@@ -563,17 +597,51 @@ __global__ void __launch_bounds__(64) k_final(const MsmJob* __restrict__ jobs, i
   const MsmJob& job = jobs[jb];
   const FqIlp f;
   Jac acc = jac_identity(f);
-  for (int w = job.windows - 1; w >= 0; --w) {
+  for (int w = job.rw - 1; w >= 0; --w) {
     for (int e = 0; e < job.c; ++e) acc = jac_dbl_wave(f, acc);
     acc = jac_add_wave(f, acc, load_jac_rec(wsum + job.win_base + w));
   }
   if (threadIdx.x == 0) store_jac(f, job.out, acc);
 }
 
+// Window table of an SRS: out[w * n + i] = 2^(c w) P_i as an affine record, w < windows (one
+// thread per point: c doublings between windows, one inversion per stored point).
+__global__ void __launch_bounds__(256) k_srs_window_table(const AffRec* __restrict__ pts, int64_t n, int c, int windows,
+                                                          AffRec* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Fq f;
+  const Aff p = unpack_aff(load_aff_raw(pts + i));
+  out[i] = pts[i];
+  Jac acc;
+  acc.x = p.x;
+  acc.y = p.y;
+  acc.z = fe_const<Fq>(f.one);
+  for (int w = 1; w < windows; ++w) {
+    for (int e = 0; e < c; ++e) acc = jac_dbl(f, acc);
+    store_aff_rec(out + (int64_t)w * n + i, jac_to_aff(f, acc));  // 2^(c w) P_i is finite: r is an odd prime
+  }
+}
+
 // ---- host ---------------------------------------------------------------------
 
 // Whether new plans split scalars with the GLV endomorphism (zk_msm_set_glv).
 static int g_glv = 1;
+
+// The bucket accumulation kernel new plans use (zk_msm_set_accumulate): bit 0 picks the
+// low-latency product schedule, bit 1 XYZZ coordinates.
+static int g_acc_variant = 0;
+
+using AccKernel = void (*)(const uint2*, const uint32_t*, const AffRec*, JacRec*, RunSummary*, int);
+
+static AccKernel acc_kernel(int variant) {
+  switch (variant & 3) {
+    case 1: return k_accumulate<FqIlp, false>;
+    case 2: return k_accumulate<FqConst, true>;
+    case 3: return k_accumulate<FqIlp, true>;
+    default: return k_accumulate<FqConst, false>;
+  }
+}
 
 int default_window(int64_t n) {
   // Signed digits: about n additions per window (2n over half as many windows with the GLV
@@ -602,6 +670,8 @@ struct MsmPlan {
   int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, segments = 0, sums_rows = 0, parts = 0, windows_total = 0;
   int64_t sort_blocks = 0;
   size_t bytes = 0;
+  void* srs = nullptr;  // the SRS handle the plan reads, if any
+  int acc_variant = 0;  // the accumulation kernel (g_acc_variant when planned)
   int32_t* digits = nullptr;
   const AffRec* points = nullptr;
   // device buffers
@@ -624,15 +694,17 @@ struct MsmLayout {
 // jobs: scalars, n, c (0 = default) and out filled in by the caller; the rest is planned here,
 // without touching the device.
 static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk, MsmLayout* L) {
+  plan->acc_variant = g_acc_variant;
   if (chunk < 0) return fail("chunk_len must be positive (or 0 for automatic)");
   int64_t blocks = 0, keys = 0, entries = 0, segs = 0, sums = 0, parts = 0, wins = 0, sort_blocks = 0, cnts = 0;
   for (auto& j : jobs) {
     if (j.n < 0 || j.n > 0x3fffffff) return fail("MSM size must be in [0, 2^30)");
     if (j.c <= 0) j.c = default_window(j.n);
     if (j.c > 20) return fail("window must be in 1..20");
-    j.glv = g_glv;
+    j.glv = j.table ? 0 : g_glv;
     j.nv = j.glv ? 2 * j.n : j.n;
     j.windows = (j.glv ? Glv::kBits : 254) / j.c + 1;
+    j.rw = j.table ? 1 : j.windows;
     j.log_seg = std::min(2, j.c - 1);
     j.log_segs = (j.c - 1) - j.log_seg;
     j.chunks = (int)(((1LL << j.log_segs) + kRedChunk - 1) / kRedChunk);
@@ -654,12 +726,12 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
     sort_blocks += (int64_t)j.windows * j.ranges * j.tiles;
     cnts += (int64_t)j.windows * nb * j.tiles;
     blocks += std::max<int64_t>(1, (j.n + 255) / 256);
-    keys += (int64_t)j.windows << (j.c - 1);
+    keys += (int64_t)j.rw << (j.c - 1);
     entries += (int64_t)j.windows * j.nv;
-    segs += (int64_t)j.windows << j.log_segs;
-    sums += (int64_t)j.windows * (j.log_segs + 1);
-    parts += (int64_t)j.windows * (j.log_segs + 1) * j.chunks;
-    wins += j.windows;
+    segs += (int64_t)j.rw << j.log_segs;
+    sums += (int64_t)j.rw * (j.log_segs + 1);
+    parts += (int64_t)j.rw * (j.log_segs + 1) * j.chunks;
+    wins += j.rw;
   }
   if (keys >= 0x7fffffff || entries >= 0xffffffffLL) return fail("MSM batch too large for 32-bit bucket keys");
   // Accumulation grid: every resident workgroup slot of the GPU once (the kernel splits the sorted
@@ -670,7 +742,9 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
     grid0 = (entries + (int64_t)chunk * kAccThreads - 1) / ((int64_t)chunk * kAccThreads);
   } else {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_accumulate, kAccThreads, 0) != hipSuccess || per_cu < 1) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, acc_kernel(plan->acc_variant), kAccThreads, 0) !=
+            hipSuccess ||
+        per_cu < 1) {
       (void)hipGetLastError();
       per_cu = 1;
     }
@@ -797,7 +871,7 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
   const int levels = (int)plan->level_max.size();
   {
     Prof prof("msm_accumulate", stream);
-    hipLaunchKernelGGL(k_accumulate, dim3((unsigned)plan->level_max[0]), dim3(kAccThreads), 0, stream, plan->sorted,
+    hipLaunchKernelGGL(acc_kernel(plan->acc_variant), dim3((unsigned)plan->level_max[0]), dim3(kAccThreads), 0, stream, plan->sorted,
                        plan->m_dev, plan->points, plan->buckets, plan->runs[0], levels == 1 ? 1 : 0);
     ZK_LAUNCH_CHECK("msm accumulate");
   }
@@ -886,14 +960,35 @@ int zk_msm_get_glv(int* on) {
   return 0;
 }
 
+int zk_msm_set_accumulate(int variant) {
+  if (variant < 0 || variant > 3) return fail("accumulation variant must be in 0..3");
+  g_acc_variant = variant;
+  return 0;
+}
+
+int zk_msm_get_accumulate(int* variant) {
+  *variant = g_acc_variant;
+  return 0;
+}
+
 int zk_msm_g1(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window, int chunk_len,
               uint64_t* out_jac, void* stream) {
   return msm_g1(points_affine, scalars, n, window, chunk_len, out_jac, (hipStream_t)stream);
 }
 
+// An SRS on the device: records [0, n) hold P_i, [n, 2n) phi(P_i), and each window table of
+// width c holds 2^(c w) P_i at base + w * n + i.
+struct SrsTable {
+  int c, windows;
+  int64_t base;
+};
+
 struct ZkSrs {
   AffRec* pts;
   int64_t n;
+  int64_t records;
+  std::vector<SrsTable> tables;
+  int plans = 0;  // live plans hold pts: no table may be added meanwhile
 };
 
 int zk_srs_create(const uint64_t* points_affine, int64_t n, void** srs, void* stream) {
@@ -905,7 +1000,41 @@ int zk_srs_create(const uint64_t* points_affine, int64_t n, void** srs, void* st
     return rc;
   }
   ZK_HIP(hipStreamSynchronize((hipStream_t)stream));
-  *srs = new ZkSrs{p, n};
+  auto* s = new ZkSrs{p, n, 2 * n, {}, 0};
+  *srs = s;
+  return 0;
+}
+
+int zk_srs_precompute(void* srs, int window, void* stream_) {
+  auto* s = static_cast<ZkSrs*>(srs);
+  hipStream_t stream = (hipStream_t)stream_;
+  if (!s) return fail("an SRS handle is required");
+  if (window < 4 || window > 20) return fail("window table width must be in 4..20");
+  if (s->plans > 0) return fail("window tables must be added before plans are created over the SRS");
+  for (const SrsTable& t : s->tables)
+    if (t.c == window) return 0;
+  const int windows = 254 / window + 1;
+  const size_t records = (size_t)s->records + (size_t)windows * (size_t)s->n;
+  if (records >= ((size_t)1 << 31)) return fail("window tables would exceed 2^31 point records (31-bit point references)");
+  AffRec* p = nullptr;
+  ZK_HIP(hipMalloc(&p, sizeof(AffRec) * records));
+  hipError_t e = hipMemcpyAsync(p, s->pts, sizeof(AffRec) * (size_t)s->records, hipMemcpyDeviceToDevice, stream);
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    return fail_hip(e, "SRS copy");
+  }
+  hipLaunchKernelGGL(k_srs_window_table, dim3(grid_for(s->n, 256)), dim3(256), 0, stream, s->pts, s->n, window, windows,
+                     p + s->records);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    return fail_hip(e, "SRS window table");
+  }
+  (void)hipFree(s->pts);
+  s->pts = p;
+  s->tables.push_back(SrsTable{window, windows, s->records});
+  s->records = (int64_t)records;
   return 0;
 }
 
@@ -918,22 +1047,47 @@ int zk_srs_destroy(void* srs) {
   return 0;
 }
 
+// The window table an MSM of n points uses: the one with the least work (entries plus
+// reduction) among the SRS's tables, or the one of the requested width; nullptr for none.
+static const SrsTable* pick_table(const ZkSrs* s, int64_t n, int window) {
+  const SrsTable* best = nullptr;
+  double best_cost = 1e300;
+  for (const SrsTable& t : s->tables) {
+    if (window > 0 && t.c != window) continue;
+    const double cost = (double)t.windows * (double)n + 3.0 * (double)(1LL << (t.c - 1));
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = &t;
+    }
+  }
+  return best;
+}
+
 int zk_msm_plan_create(void* srs, int count, const uint64_t* const* scalars, const int64_t* n, const int32_t* window,
                        int chunk_len, uint64_t* out_jac, void** plan, void* stream) {
   auto* s = static_cast<ZkSrs*>(srs);
   if (!s) return fail("an SRS handle is required");
   std::vector<MsmJob> jobs((size_t)std::max(count, 0));
   for (int j = 0; j < count; ++j) {
-    if (n[j] > s->n) return fail("MSM size exceeds the SRS");
-    jobs[(size_t)j] = MsmJob{};
-    jobs[(size_t)j].scalars = scalars[j];
-    jobs[(size_t)j].phi_base = s->n;
-    jobs[(size_t)j].n = n[j];
-    jobs[(size_t)j].c = window ? window[j] : 0;
-    jobs[(size_t)j].out = out_jac + 12 * j;
+    if (n[j] < 0 || n[j] > s->n) return fail("MSM size exceeds the SRS");
+    MsmJob& job = jobs[(size_t)j];
+    job = MsmJob{};
+    job.scalars = scalars[j];
+    job.phi_base = s->n;
+    job.n = n[j];
+    job.c = window ? window[j] : 0;
+    job.out = out_jac + 12 * j;
+    if (const SrsTable* t = pick_table(s, n[j], job.c)) {
+      job.table = 1;
+      job.c = t->c;
+      job.table_base = t->base;
+      job.table_stride = s->n;
+    }
   }
   MsmPlan* p = nullptr;
   if (int rc = msm_plan_create(s->pts, jobs, chunk_len, (hipStream_t)stream, &p)) return rc;
+  p->srs = s;
+  ++s->plans;
   *plan = p;
   return 0;
 }
@@ -955,7 +1109,9 @@ int zk_msm_workspace_bytes(int count, const int64_t* n, const int32_t* window, i
 int zk_msm_plan_launch(void* plan, void* stream) { return msm_plan_launch(static_cast<MsmPlan*>(plan), (hipStream_t)stream); }
 
 int zk_msm_plan_destroy(void* plan) {
-  msm_plan_destroy(static_cast<MsmPlan*>(plan));
+  auto* p = static_cast<MsmPlan*>(plan);
+  if (p && p->srs) --static_cast<ZkSrs*>(p->srs)->plans;
+  msm_plan_destroy(p);
   return 0;
 }
 

@@ -120,7 +120,43 @@ __device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64
   }
   __syncthreads();
 
-  for (int t = 0; t < geo.k; ++t) {
+  // Stages in pairs (radix 4): a thread takes rows m, m + H/2, m + H, m + 3H/2 of a group through
+  // stage t (pairs at distance H = 2^(k-1-t)) and stage t + 1 (distance H/2) in registers:
+  // 4 products, 3 twiddles, one LDS round trip and one barrier for two stages.
+  int t = 0;
+  for (; t + 1 < geo.k; t += 2) {
+    const int hb = geo.k - 1 - t;  // >= 1
+    const int h2 = 1 << (hb - 1);
+    const int s = geo.s0 + t;
+    const uint64_t* tw_a = d.tw + 4 * (n - (n >> s));        // stage s
+    const uint64_t* tw_b = d.tw + 4 * (n - (n >> (s + 1)));  // stage s + 1
+    const bool last_b = s + 1 == geo.log_n - 1;             // stage s + 1 has the single twiddle 1
+    for (int u = threadIdx.x; u < (T >> 2); u += kThreads) {
+      const int g = u & (C - 1), qv = u >> geo.log_c;
+      const int mlow = qv & (h2 - 1);
+      const int m00 = ((qv >> (hb - 1)) << (hb + 1)) | mlow;
+      const int64_t lo = geo.row(wg, g, m00) & ((1LL << log_s) - 1);
+      const int64_t ja = ((int64_t)mlow << log_s) + lo, jb = ((int64_t)(mlow + h2) << log_s) + lo;
+      const int sl0 = (m00 << geo.log_c) + g, step = h2 << geo.log_c;
+      const Fe x0 = lds_get(lds, sl0), x1 = lds_get(lds, sl0 + step), x2 = lds_get(lds, sl0 + 2 * step),
+               x3 = lds_get(lds, sl0 + 3 * step);
+      const Fe y0 = fadd_lazy(f, x0, x2), y1 = fadd_lazy(f, x1, x3);
+      const Fe y2 = mont_raw(f, fsub_for_mul(f, x0, x2), load_row(tw_a + 4 * ja));
+      const Fe y3 = mont_raw(f, fsub_for_mul(f, x1, x3), load_row(tw_a + 4 * jb));
+      lds_put(lds, sl0, fadd_lazy(f, y0, y1));
+      lds_put(lds, sl0 + 2 * step, fadd_lazy(f, y2, y3));
+      if (last_b) {
+        lds_put(lds, sl0 + step, fsub_lazy(f, y0, y1));
+        lds_put(lds, sl0 + 3 * step, fsub_lazy(f, y2, y3));
+      } else {
+        const Fe wc = load_row(tw_b + 4 * ja);
+        lds_put(lds, sl0 + step, mont_raw(f, fsub_for_mul(f, y0, y1), wc));
+        lds_put(lds, sl0 + 3 * step, mont_raw(f, fsub_for_mul(f, y2, y3), wc));
+      }
+    }
+    __syncthreads();
+  }
+  for (; t < geo.k; ++t) {  // an odd stage count ends with one radix-2 stage
     const int hb = geo.k - 1 - t;  // bit of m that this stage pairs
     const int s = geo.s0 + t;
     const uint64_t* tws = d.tw + 4 * (n - (n >> s));  // stage s table: n >> (s + 1) entries

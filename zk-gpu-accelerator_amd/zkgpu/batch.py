@@ -47,6 +47,17 @@ class SRS:
         _gb.call("zk_srs_create", self.points.ptr, self.n, ctypes.byref(h), _gb.current_stream())
         self.handle = h.value
 
+    def precompute(self, windows: Sequence[int] = (16,)) -> SRS:
+        """
+        Add window tables (2^(c w) P_i for every window of width c): MSMs planned afterwards put
+        all windows' digits into one set of buckets, with no window combination. W x 64 bytes per
+        point each (W = floor(254 / c) + 1); add them before creating plans.
+        """
+        for c in windows:
+            _gb.call("zk_srs_precompute", self.handle, int(c), _gb.current_stream())
+        self.tables = sorted(set(getattr(self, "tables", [])) | {int(c) for c in windows})
+        return self
+
     @classmethod
     def from_keys(cls, keys: np.ndarray) -> SRS:
         """The basis k_i * G for plain (n, 4) scalar rows k_i, generated on the GPU."""

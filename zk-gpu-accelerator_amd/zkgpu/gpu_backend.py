@@ -88,6 +88,8 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_msm_default_window": [_i64, ctypes.POINTER(_i32)],
     "zk_msm_set_glv": [_i32],
     "zk_msm_get_glv": [ctypes.POINTER(_i32)],
+    "zk_msm_set_accumulate": [_i32],
+    "zk_msm_get_accumulate": [ctypes.POINTER(_i32)],
     "zk_prof_enable": [_i32],
     "zk_prof_read": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), _i64p, _i32],
     "zk_field_register": [_u64p, ctypes.POINTER(_i32)],
@@ -113,6 +115,7 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_msm_g1": [_vp, _vp, _i64, _i32, _i32, _vp, _vp],
     "zk_srs_create": [_vp, _i64, ctypes.POINTER(_vp), _vp],
     "zk_srs_destroy": [_vp],
+    "zk_srs_precompute": [_vp, _i32, _vp],
     "zk_msm_plan_create": [_vp, _i32, _vp, _vp, _vp, _i32, _vp, ctypes.POINTER(_vp), _vp],
     "zk_msm_plan_launch": [_vp, _vp],
     "zk_msm_plan_destroy": [_vp],
