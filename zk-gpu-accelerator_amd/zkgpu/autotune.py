@@ -10,12 +10,19 @@ Three tables, each keyed by log2 of the problem size:
                bucket accumulation; 0 = one persistent wave of workgroups over
                the whole GPU);
 - ntt_fusion:  radix-2 stages fused per LDS-staged NTT pass (which fixes the
-               number of HBM sweeps and the columns per tile).
+               number of HBM sweeps and the columns per tile);
+
+and one device-wide choice:
+
+- msm_accumulate: the bucket accumulation kernel (Jacobian or XYZZ bucket
+               coordinates, serial or low-latency products; zk_msm_set_accumulate),
+               measured on the largest MSM.
 
 tune() measures every candidate with HIP events (median of several runs) and
 writes zkgpu/tuning/<device>.json; lookup() reads the table of the GPU in use,
-and apply() hands the NTT fusion depths to the library (gpu_backend calls it
-once the device is up). The reference picks its MSM window from a closed-form
+and apply() hands the NTT fusion depths and the accumulation kernel to the
+library (gpu_backend calls it once the device is up). MSMs over an SRS with
+window tables take their window from the table chosen per job instead. The reference picks its MSM window from a closed-form
 rule (src/zkgpu/msm.py:33-50) and has no NTT fusion at all; that rule stays the
 fallback when no table exists for this GPU.
 
@@ -74,7 +81,7 @@ def lookup(kind: str, n: int, device: str | None = None) -> int:
 
 
 def apply(device: str | None = None) -> int:
-    """Hand the tuned NTT fusion depths to the library; returns how many sizes were set."""
+    """Hand the tuned NTT fusion depths and accumulation kernel to the library; returns how many were set."""
     from zkgpu import gpu_backend as gb
 
     table = load(device)
@@ -83,6 +90,9 @@ def apply(device: str | None = None) -> int:
     count = 0
     for k, v in table.get("ntt_fusion", {}).items():
         gb.call("zk_ntt_set_fusion", int(k), int(v))
+        count += 1
+    if "msm_accumulate" in table:
+        gb.call("zk_msm_set_accumulate", int(table["msm_accumulate"]))
         count += 1
     return count
 
@@ -141,7 +151,7 @@ def tune_ntt(log_sizes: range, reps: int, log) -> tuple[dict[str, int], dict[str
     return best, seen
 
 
-def tune_msm(log_sizes: range, reps: int, log) -> tuple[dict[str, int], dict[str, int], dict[str, Any]]:
+def tune_msm(log_sizes: range, reps: int, log) -> tuple[dict[str, int], dict[str, int], int, dict[str, Any]]:
     from zkgpu import gpu_backend as gb
     from zkgpu.batch import SRS, MsmBatch
     from zkgpu.workloads import key_rows, srs_points, witness_rows_fast
@@ -152,6 +162,7 @@ def tune_msm(log_sizes: range, reps: int, log) -> tuple[dict[str, int], dict[str
     windows: dict[str, int] = {}
     chunks: dict[str, int] = {}
     seen: dict[str, Any] = {}
+    accumulate = 0
     for lg in log_sizes:
         n = 1 << lg
         c0 = ctypes.c_int()
@@ -170,11 +181,22 @@ def tune_msm(log_sizes: range, reps: int, log) -> tuple[dict[str, int], dict[str
         k_best = min(chunk_res, key=chunk_res.get)
         windows[str(lg)] = c_best
         chunks[str(lg)] = k_best
+        if lg == max(log_sizes):
+            var_res = {}
+            for v in range(4):
+                gb.call("zk_msm_set_accumulate", v)
+                plan = MsmBatch(srs, [pool[:n]], windows=[c_best], chunk_len=k_best)
+                var_res[v] = _median_ms(plan.launch, reps)
+                del plan
+            accumulate = min(var_res, key=var_res.get)
+            gb.call("zk_msm_set_accumulate", accumulate)
+            seen["accumulate_ms"] = {str(v): round(t, 4) for v, t in var_res.items()}
+            log(f"msm 2^{lg}: accumulation variant {accumulate} " + ", ".join(f"{v}:{t:.3f}ms" for v, t in var_res.items()))
         seen[str(lg)] = {"window_ms": {str(c): round(v, 4) for c, v in results.items()},
                          "chunk_ms": {str(k): round(v, 4) for k, v in chunk_res.items()}}
         log(f"msm 2^{lg}: window {c_best} ({results[c_best]:.3f} ms; model {c0.value}: {results.get(c0.value, float('nan')):.3f} ms), "
             f"chunk {k_best} ({chunk_res[k_best]:.3f} ms)")
-    return windows, chunks, seen
+    return windows, chunks, accumulate, seen
 
 
 def tune(*, quick: bool = False, write: bool = True, log=print) -> dict[str, Any]:
@@ -186,7 +208,7 @@ def tune(*, quick: bool = False, write: bool = True, log=print) -> dict[str, Any
     msm_sizes = range(10, 21, 2 if quick else 1)
     t0 = time.time()
     fusion, ntt_ms = tune_ntt(ntt_sizes, reps, log)
-    win, chunk, msm_ms = tune_msm(msm_sizes, reps, log)
+    win, chunk, accumulate, msm_ms = tune_msm(msm_sizes, reps, log)
     table = {
         "device": device_key(),
         "created": time.strftime("%Y-%m-%d %H:%M:%S"),
@@ -194,6 +216,7 @@ def tune(*, quick: bool = False, write: bool = True, log=print) -> dict[str, Any
         "msm_window": win,
         "msm_chunk": chunk,
         "ntt_fusion": fusion,
+        "msm_accumulate": accumulate,
         "measured_ms": {"ntt": ntt_ms, "msm": msm_ms},
     }
     if write:
@@ -211,7 +234,7 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--no-write", action="store_true", help="print the table without writing it")
     args = ap.parse_args(argv)
     table = tune(quick=args.quick, write=not args.no_write, log=lambda *a: print(*a, file=sys.stderr, flush=True))
-    print(json.dumps({k: table[k] for k in ("device", "msm_window", "msm_chunk", "ntt_fusion")}))
+    print(json.dumps({k: table[k] for k in ("device", "msm_window", "msm_chunk", "ntt_fusion", "msm_accumulate")}))
     return 0
 
 

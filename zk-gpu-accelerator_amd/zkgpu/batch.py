@@ -99,7 +99,12 @@ class MsmBatch:
         if any(n > srs.n for n in self.sizes):
             raise ValueError(f"an MSM is larger than the SRS ({srs.n} points)")
         count = len(self.scalars)
-        self.windows = list(windows) if windows is not None else [_tuned_window(n) for n in self.sizes]
+        if windows is not None:
+            self.windows = list(windows)
+        elif getattr(srs, "tables", None):
+            self.windows = [0] * count  # the library picks each job's window table
+        else:
+            self.windows = [_tuned_window(n) for n in self.sizes]
         if chunk_len is None:
             chunk_len = _tuned_chunk(self.sizes)
         self.out = _gb.empty((max(count, 1), 3, 4))
