@@ -159,6 +159,7 @@ def main() -> int:
     traffic = _traffic("k_accumulate", n)
     extras: dict = {}
     if not args.no_extras:
+        extras.update(_msm_2p16(args, gb, workloads, msm_g1_jacobian, log))
         extras.update(_ntt_and_pipeline(args, gb, BN254_FR, workloads, log))
         from zkgpu.batch import SRS
 
@@ -211,6 +212,17 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _msm_2p16(args, gb, workloads, msm_g1_jacobian, log) -> dict:
+    """The plain MSM at 2^16 points (P_i = k_i G, k_i seeded 0x5EED02, witness-like scalars)."""
+    n = 1 << 16
+    pts = workloads.srs_points(workloads.key_rows(n, 0x5EED02))
+    sc = gb.asarray(workloads.witness_rows(n, 0x5EED02))
+    out = gb.empty((3, 4))
+    ms = timed(lambda: msm_g1_jacobian(pts, sc, out), args.steps, 2, gb)
+    log(f"msm 2^16: {ms:.3f} ms, {n / (ms * 1e-3) / 1e6:.2f} Mpts/s")
+    return {"msm_2p16": {"ms": round(ms, 4), "mpts_s": round(n / (ms * 1e-3) / 1e6, 3)}}
 
 
 def _traffic(kernel: str, n: int) -> dict | None:
@@ -376,7 +388,8 @@ def _batched(args, gb, points, srs, workloads, log) -> dict:
             "per_call_ms": round(plain_ms, 3),
             "batched_graph_ms": round(batch_ms, 3),
             "speedup": round(plain_ms / batch_ms, 3),
-            "windows": msm.windows[:4] + ["..."],
+            "msm_windows": "per job: the SRS window table (widths %s) with the least work for its size"
+                           % ", ".join(str(c) for c in SRS_TABLES),
         }
     }
 
