@@ -382,12 +382,30 @@ def _batched(args, gb, points, srs, workloads, log) -> dict:
         if (got[k].is_identity() != want.is_identity()) or (not want.is_identity() and got[k].affine() != want.affine()):
             raise RuntimeError(f"bench: batched MSM {k} differs from the per-call result")
     log(f"batched: per-call {plain_ms:.2f} ms, batched graph {batch_ms:.2f} ms, speedup {plain_ms / batch_ms:.2f}x")
+    # baseline (i): the reference's public API, Python ints in (msm_pippenger_gpu), on a sample of
+    # the MSMs, scaled to the workload by point count
+    from zkgpu.field256_gpu import BN254_FQ, _limbs_to_ints
+    from zkgpu.msm_gpu import msm_pippenger_gpu
+
+    sample = [k for k in range(len(msm_sizes)) if msm_sizes[k] <= (1 << 16)][:6]
+    coords = _limbs_to_ints(BN254_FQ.to_limbs(points[: max(msm_sizes[k] for k in sample)].reshape(-1, 4)).get())
+    pool_host = _limbs_to_ints(pool[: max(msm_sizes[k] for k in sample)].get())
+    t0 = time.perf_counter()
+    for k in sample:
+        m = msm_sizes[k]
+        msm_pippenger_gpu(coords[0 : 2 * m : 2], coords[1 : 2 * m : 2], pool_host[:m])
+    ints_s = time.perf_counter() - t0
+    ints_ms_total = ints_s * 1e3 * sum(msm_sizes) / sum(msm_sizes[k] for k in sample)
+    log(f"batched: public API with Python ints, {len(sample)} sampled MSMs: {ints_s:.2f} s; scaled {ints_ms_total:.0f} ms")
     return {
         "batched": {
             "workload": "64 G1 MSMs log-uniform in [2^10, 2^20] over one 2^20 SRS + 32 Fr NTTs 2^10..2^20",
             "per_call_ms": round(plain_ms, 3),
             "batched_graph_ms": round(batch_ms, 3),
             "speedup": round(plain_ms / batch_ms, 3),
+            "per_call_python_ints_ms_est": round(ints_ms_total, 1),
+            "per_call_python_ints_note": f"msm_pippenger_gpu with Python ints on {len(sample)} of the MSMs "
+                                         "(sizes up to 2^16), scaled by point count; the NTTs excluded",
             "msm_windows": "per job: the SRS window table (widths %s) with the least work for its size"
                            % ", ".join(str(c) for c in SRS_TABLES),
         }

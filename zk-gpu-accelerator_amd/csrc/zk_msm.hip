@@ -540,8 +540,17 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
   Jac acc = jac_identity(f);
   const JacRec* src = (j == job.log_segs ? tot : run) + job.seg_base + win * segs;
   const int64_t lo = chunk * kRedChunk, hi = min(segs, lo + (int64_t)kRedChunk);
-  for (int64_t s = lo + threadIdx.x; s < hi; s += 256) {
-    if (j == job.log_segs || ((s >> j) & 1)) acc = jac_add(f, acc, load_jac_rec(src + s));
+  if (j == job.log_segs || ((1LL << j) >= kRedChunk && ((lo >> j) & 1))) {
+    // the T row, or a bit that is set across the whole chunk (chunks are 2^11-aligned)
+    for (int64_t s = lo + threadIdx.x; s < hi; s += 256) acc = jac_add(f, acc, load_jac_rec(src + s));
+  } else if ((1LL << j) < kRedChunk) {
+    // only the segments with bit j set: half of [lo, hi), whose ends are multiples of 2^(j+1), so
+    // that neighbouring lanes never skip adds the wavefront executes anyway
+    const int64_t low = (1LL << j) - 1;
+    for (int64_t k = threadIdx.x; k < (hi - lo) >> 1; k += 256) {
+      const int64_t s = lo + ((k >> j) << (j + 1)) + (1LL << j) + (k & low);
+      acc = jac_add(f, acc, load_jac_rec(src + s));
+    }
   }
   store_jac_rec(&sh[threadIdx.x], acc);
   __syncthreads();
