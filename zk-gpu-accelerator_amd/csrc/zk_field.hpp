@@ -268,7 +268,7 @@ __device__ __forceinline__ Fe fsub(const F& f, const Fe& a, const Fe& b) {
 }
 
 // ---- lazily reduced arithmetic (for long chains of additions and products, e.g. NTT passes) ----
-// Values are kept in [0, 2p). mont_raw of operands below 4p and p stays below 2p (4p * p
+// Values are kept in [0, 2p). mont_raw of operands below 6p and p stays below 2p (6p * p
 // < p * 2^261), so a product needs no final subtraction.
 
 // (a + b) reduced into [0, 2p), for a, b < 2p.
@@ -294,10 +294,30 @@ __device__ __forceinline__ Fe fadd_lazy(const F& f, const Fe& a, const Fe& b) {
   return s;
 }
 
-// a - b + 2p, in (0, 4p) for a, b < 2p, unreduced: the operand of a product.
+// Built-in fields carry p4r: 4p with limbs 1..7 raised by 2^29 - 1 and limb 0 by 2^29, the
+// borrows taken from the limb above (the top limb is 1 lower).
+template <class F, class = void>
+struct HasP4r {
+  static constexpr bool value = false;
+};
+template <class F>
+struct HasP4r<F, decltype((void)F::p4r)> {
+  static constexpr bool value = true;
+};
+
+// The operand of a product: a - b + 4p in (2p, 6p) with limbs below 2^31 and no carries
+// (fields with p4r), else a - b + 2p in (0, 4p) with normalized limbs; for a, b < 2p.
+// Against p4r each limb a_i + p4r_i - b_i lies in [0, 3 * 2^29). mont_raw takes such an
+// operand: a column then sums at most 9 * 2^60 + 9 * 2^58 + 2^35 < 2^64, and the result
+// (a b + m p) / 2^261 < 6p^2 / 2^261 + p stays below 2p.
 template <class F>
 __device__ __forceinline__ Fe fsub_for_mul(const F& f, const Fe& a, const Fe& b) {
   Fe d;
+  if constexpr (HasP4r<F>::value) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) d.l[i] = a.l[i] + F::p4r[i] - b.l[i];
+    return d;
+  }
   int32_t c = 0;
 #pragma unroll
   for (int i = 0; i < NL; ++i) {
@@ -370,20 +390,37 @@ __device__ __forceinline__ void to_words(const Fe& a, uint32_t w[8]) {
   }
 }
 
+// Rows live in device memory. Pointers that reach a kernel through a descriptor in memory are
+// generic to the compiler, which would then issue flat accesses (and make every LDS wait also
+// wait for them); the row accessors address the global space explicitly.
+template <class T>
+using GlobalPtr = __attribute__((address_space(1))) T*;
+using U32x4 = uint32_t __attribute__((ext_vector_type(4)));  // a 16-byte access
+
+template <class T>
+__device__ __forceinline__ GlobalPtr<const T> global_ptr(const void* p) {
+  return (GlobalPtr<const T>)p;
+}
+
+template <class T>
+__device__ __forceinline__ GlobalPtr<T> global_ptr(void* p) {
+  return (GlobalPtr<T>)p;
+}
+
 // Loads one 32-byte row (two 16-byte loads).
 __device__ __forceinline__ Fe load_row(const uint64_t* row) {
-  const uint4* p = reinterpret_cast<const uint4*>(row);
-  uint4 x = p[0], y = p[1];
-  uint32_t w[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+  const GlobalPtr<const U32x4> p = global_ptr<U32x4>(row);
+  const U32x4 x = p[0], y = p[1];
+  uint32_t w[8] = {x[0], x[1], x[2], x[3], y[0], y[1], y[2], y[3]};
   return from_words(w);
 }
 
 __device__ __forceinline__ void store_row(uint64_t* row, const Fe& a) {
   uint32_t w[8];
   to_words(a, w);
-  uint4* p = reinterpret_cast<uint4*>(row);
-  p[0] = make_uint4(w[0], w[1], w[2], w[3]);
-  p[1] = make_uint4(w[4], w[5], w[6], w[7]);
+  const GlobalPtr<U32x4> p = global_ptr<U32x4>(row);
+  p[0] = U32x4{w[0], w[1], w[2], w[3]};
+  p[1] = U32x4{w[4], w[5], w[6], w[7]};
 }
 
 // Canonical plain value < 2^256 from raw 32-bit words, reduced mod p, in the 256-form.

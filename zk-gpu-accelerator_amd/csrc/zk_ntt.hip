@@ -19,14 +19,17 @@
 // stages: three read+write sweeps of HBM instead of the reference's 23.
 //
 // Reduction. Inside a pass values stay in [0, 2p): a sum is brought back under
-// 2p, a difference u - v + 2p goes straight into the product, whose result is
-// below 2p with no final subtraction; intermediate passes store such values,
+// 2p, a difference u - v + 4p (limbs kept redundant, no carries) goes straight
+// into the product, whose result is below 2p with no final subtraction
+// (zk_field.hpp, fsub_for_mul); intermediate passes store such values,
 // the last pass stores canonical ones. The last stage of a transform has the
 // single twiddle 1 and does no product.
 //
 // Twiddles. One table per stage, w_s[j] = omega^(j * 2^s), so that a pass
 // reads each stage's twiddles contiguously; values are in the 261-form, so
-// (u - v) * w keeps the data's 256-form without conversions. Coset powers and
+// (u - v) * w keeps the data's 256-form without conversions. Stage entries are
+// 48-byte records of the nine limbs themselves (no repacking after the load);
+// the other tables hold 32-byte rows like the data. Coset powers and
 // n^-1 are tables in the same form, applied as the first pass loads and the
 // last pass stores. Tables are built on the device from two short seed tables
 // (host powers of omega) and cached per (field, log n, omega, coset).
@@ -55,6 +58,20 @@ __device__ __forceinline__ Fe lds_get(const uint32_t* lds, int slot) {
   Fe x;
 #pragma unroll
   for (int i = 0; i < NL; ++i) x.l[i] = lds[i * kTile + slot];
+  return x;
+}
+
+// Stage-twiddle records: nine 29-bit limbs and three zero words, 48 bytes (16-byte aligned).
+constexpr int kTwWords = 6;  // 64-bit words per record
+
+__device__ __forceinline__ Fe load_tw(const uint64_t* tab, int64_t j) {
+  const GlobalPtr<const U32x4> p = global_ptr<U32x4>(tab + kTwWords * j);
+  const U32x4 a = p[0], b = p[1];
+  const uint32_t c = global_ptr<uint32_t>(tab + kTwWords * j + 4)[0];
+  Fe x;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) x.l[i] = a[i], x.l[4 + i] = b[i];
+  x.l[8] = c;
   return x;
 }
 
@@ -128,8 +145,8 @@ __device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64
     const int hb = geo.k - 1 - t;  // >= 1
     const int h2 = 1 << (hb - 1);
     const int s = geo.s0 + t;
-    const uint64_t* tw_a = d.tw + 4 * (n - (n >> s));        // stage s
-    const uint64_t* tw_b = d.tw + 4 * (n - (n >> (s + 1)));  // stage s + 1
+    const uint64_t* tw_a = d.tw + kTwWords * (n - (n >> s));        // stage s
+    const uint64_t* tw_b = d.tw + kTwWords * (n - (n >> (s + 1)));  // stage s + 1
     const bool last_b = s + 1 == geo.log_n - 1;             // stage s + 1 has the single twiddle 1
     for (int u = threadIdx.x; u < (T >> 2); u += kThreads) {
       const int g = u & (C - 1), qv = u >> geo.log_c;
@@ -141,15 +158,15 @@ __device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64
       const Fe x0 = lds_get(lds, sl0), x1 = lds_get(lds, sl0 + step), x2 = lds_get(lds, sl0 + 2 * step),
                x3 = lds_get(lds, sl0 + 3 * step);
       const Fe y0 = fadd_lazy(f, x0, x2), y1 = fadd_lazy(f, x1, x3);
-      const Fe y2 = mont_raw(f, fsub_for_mul(f, x0, x2), load_row(tw_a + 4 * ja));
-      const Fe y3 = mont_raw(f, fsub_for_mul(f, x1, x3), load_row(tw_a + 4 * jb));
+      const Fe y2 = mont_raw(f, fsub_for_mul(f, x0, x2), load_tw(tw_a, ja));
+      const Fe y3 = mont_raw(f, fsub_for_mul(f, x1, x3), load_tw(tw_a, jb));
       lds_put(lds, sl0, fadd_lazy(f, y0, y1));
       lds_put(lds, sl0 + 2 * step, fadd_lazy(f, y2, y3));
       if (last_b) {
         lds_put(lds, sl0 + step, fsub_lazy(f, y0, y1));
         lds_put(lds, sl0 + 3 * step, fsub_lazy(f, y2, y3));
       } else {
-        const Fe wc = load_row(tw_b + 4 * ja);
+        const Fe wc = load_tw(tw_b, ja);
         lds_put(lds, sl0 + step, mont_raw(f, fsub_for_mul(f, y0, y1), wc));
         lds_put(lds, sl0 + 3 * step, mont_raw(f, fsub_for_mul(f, y2, y3), wc));
       }
@@ -159,7 +176,7 @@ __device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64
   for (; t < geo.k; ++t) {  // an odd stage count ends with one radix-2 stage
     const int hb = geo.k - 1 - t;  // bit of m that this stage pairs
     const int s = geo.s0 + t;
-    const uint64_t* tws = d.tw + 4 * (n - (n >> s));  // stage s table: n >> (s + 1) entries
+    const uint64_t* tws = d.tw + kTwWords * (n - (n >> s));  // stage s table: n >> (s + 1) entries
     const bool last_stage = s == geo.log_n - 1;        // its only twiddle is 1
     for (int b = threadIdx.x; b < (T >> 1); b += kThreads) {
       const int g = b & (C - 1), qv = b >> geo.log_c;
@@ -174,7 +191,7 @@ __device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64
       if (last_stage) {
         lds_put(lds, s1slot, fsub_lazy(f, u, v));  // w = omega^0 = 1
       } else {
-        Fe w = load_row(tws + 4 * j);
+        Fe w = load_tw(tws, j);
         lds_put(lds, s1slot, mont_raw(f, fsub_for_mul(f, u, v), w));
       }
     }
@@ -214,11 +231,12 @@ __global__ void __launch_bounds__(kThreads) k_ntt_pass_multi(F f, const NttPassD
 }
 
 // Table expansion: out[j] = seed_lo[e & mask] * seed_hi[e >> h] for e = j << shift, in the 261-form.
-// seed tables hold plain values; c522 lifts them to the 261-form.
+// seed tables hold plain values; c522 lifts them to the 261-form. Entries are 32-byte rows, or
+// 48-byte stage-twiddle records when records is set.
 template <class F>
 __global__ void __launch_bounds__(256) k_expand(F f, const uint64_t* __restrict__ lo_tab, const uint64_t* __restrict__ hi_tab,
                                                 int h, int shift, uint64_t* __restrict__ out, int64_t count,
-                                                const uint64_t* __restrict__ scale) {
+                                                const uint64_t* __restrict__ scale, int records) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
   const int64_t e = j << shift;
@@ -227,7 +245,14 @@ __global__ void __launch_bounds__(256) k_expand(F f, const uint64_t* __restrict_
   Fe b = mont(f, load_row(hi_tab + 4 * (e >> h)), lift);
   Fe x = mont(f, a, b);
   if (scale) x = mont(f, x, mont(f, load_row(scale), lift));
-  store_row(out + 4 * j, x);
+  if (records) {
+    const GlobalPtr<U32x4> p = global_ptr<U32x4>(out + kTwWords * j);
+    p[0] = U32x4{x.l[0], x.l[1], x.l[2], x.l[3]};
+    p[1] = U32x4{x.l[4], x.l[5], x.l[6], x.l[7]};
+    p[2] = U32x4{x.l[8], 0u, 0u, 0u};
+  } else {
+    store_row(out + 4 * j, x);
+  }
 }
 
 // ---- host side -----------------------------------------------------------
@@ -267,7 +292,7 @@ bool field_modulus(int field_id, U256* p) {
 
 // Expands count entries of base^(j << shift) (times scale, if given) into out, on the device.
 int expand(int field_id, const HostField& hf, const U256& base_plain, int log_count, int shift, uint64_t* out, int64_t count,
-           const U256* scale_plain, hipStream_t stream) {
+           const U256* scale_plain, bool records, hipStream_t stream) {
   const int bits = log_count + shift;  // exponents are below 2^bits
   const int h = (bits + 1) / 2;
   const int64_t nlo = 1LL << h, nhi = (1LL << (bits - h)) + 1;
@@ -292,7 +317,7 @@ int expand(int field_id, const HostField& hf, const U256& base_plain, int log_co
   ZK_HIP(hipMemcpyAsync(dev, host.data(), host.size() * 8, hipMemcpyHostToDevice, stream));
   int rc = with_field(field_id, [&](auto f) {
     hipLaunchKernelGGL((k_expand<decltype(f)>), dim3(grid_for(count, 256)), dim3(256), 0, stream, f, dev, dev + 4 * nlo, h,
-                       shift, out, count, scale_plain ? dev + 4 * (nlo + nhi) : nullptr);
+                       shift, out, count, scale_plain ? dev + 4 * (nlo + nhi) : nullptr, records ? 1 : 0);
     ZK_LAUNCH_CHECK("table expansion");
     return 0;
   });
@@ -317,29 +342,29 @@ int get_table(const TableKey& key, hipStream_t stream, const uint64_t** out) {
   const int64_t n = 1LL << key.log_n;
   size_t rows = key.kind == 0 ? (size_t)(n > 1 ? n - 1 : 1) : key.kind >= 3 ? 1 : (size_t)n;
   uint64_t* buf = nullptr;
-  ZK_HIP(hipMalloc(&buf, rows * 32));
+  ZK_HIP(hipMalloc(&buf, rows * (key.kind == 0 ? 8 * kTwWords : 32)));
   int rc = 0;
   if (key.kind == 0) {
     U256 omega = U256::from(key.a);
     for (int s = 0; s < key.log_n && rc == 0; ++s) {
       const int64_t off = n - (n >> s), cnt = n >> (s + 1);
-      rc = expand(key.field, hf, omega, key.log_n - 1 - s, s, buf + 4 * off, cnt, nullptr, stream);
+      rc = expand(key.field, hf, omega, key.log_n - 1 - s, s, buf + kTwWords * off, cnt, nullptr, true, stream);
     }
   } else if (key.kind == 1) {
-    rc = expand(key.field, hf, U256::from(key.b), key.log_n, 0, buf, n, nullptr, stream);
+    rc = expand(key.field, hf, U256::from(key.b), key.log_n, 0, buf, n, nullptr, false, stream);
   } else if (key.kind == 4) {
     U256 value = hf.reduce(U256::from(key.a));
-    rc = expand(key.field, hf, U256::of(1), 0, 0, buf, 1, &value, stream);
+    rc = expand(key.field, hf, U256::of(1), 0, 0, buf, 1, &value, false, stream);
   } else if (key.kind == 3) {
     U256 scale = hf.mulp(hf.invp(U256::of((uint64_t)n)), U256::from(key.a));
-    rc = expand(key.field, hf, U256::of(1), 0, 0, buf, 1, &scale, stream);
+    rc = expand(key.field, hf, U256::of(1), 0, 0, buf, 1, &scale, false, stream);
   } else {
     // extra * n^-1 * g^-j
     U256 g = U256::from(key.b);
     U256 ginv = hf.invp(g);
     U256 ninv = hf.invp(U256::of((uint64_t)n));
     U256 scale = hf.mulp(ninv, U256::from(key.a));  // a carries an extra factor (1 or 2^5)
-    rc = expand(key.field, hf, ginv, key.log_n, 0, buf, n, &scale, stream);
+    rc = expand(key.field, hf, ginv, key.log_n, 0, buf, n, &scale, false, stream);
   }
   if (rc) {
     (void)hipFree(buf);
