@@ -28,13 +28,17 @@
 //              consecutive entries per thread, added to an XYZZ accumulator with
 //              affine points (madd-2008-s: 10 products instead of Jacobian's 11);
 //              a run of equal keys strictly inside a thread's range is a whole
-//              bucket and is written at once, as a Jacobian point. The runs cut by range
+//              bucket and is written at once, in the accumulator's coordinates (no
+//              conversion inside the loop: some lane of a wave ends a run in most
+//              iterations, so a conversion there would cost the whole wave 4 products
+//              per entry; the reduction converts each bucket once). The runs cut by range
 //              boundaries are merged by a tree of run summaries (workgroup trees in
 //              LDS, then levels of 256), each merge at most one addition. The load
 //              is balanced whatever the bucket sizes, with no host sync (the
 //              reference splits buckets into chunks level by level and syncs the
 //              host per level, msm_gpu.py:287-333).
-//  5. reduce   per window, sum_v v * B_v: threads take segments of L buckets with
+//  5. reduce   per window, sum_v v * B_v: threads take segments of L buckets (each
+//              converted to Jacobian coordinates as it is loaded) with
 //              running sums (T = sum (i+1) B, R = sum B); a window's total is
 //              sum_s T_s + L * sum_j 2^j Q_j with Q_j = sum of R_s over segments s
 //              whose index has bit j set, each Q_j a block-wide tree sum.
@@ -125,6 +129,75 @@ __device__ __forceinline__ void store_jac_rec(JacRec* p, const Jac& a) {
   uint4* q = reinterpret_cast<uint4*>(p);
 #pragma unroll
   for (int i = 0; i < 7; ++i) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
+// A bucket sum: XYZZ coordinates (x, y, zz, zzz: 29-bit limbs, words 0..35) as the accumulation
+// leaves them, or a Jacobian point (x, y, z in words 0..26) marked by bit 31 of word 35 (the top
+// limb of a coordinate below 2p never has it). All-zero words are the XYZZ identity (zz = 0), so a
+// cleared bucket that receives no entries reads as the identity.
+struct alignas(16) BucketRec {
+  uint32_t w[36];
+};
+constexpr uint32_t kJacMark = 0x80000000u;
+
+__device__ __forceinline__ void store_bucket_words(BucketRec* p, const uint32_t (&w)[36]) {
+  uint4* q = reinterpret_cast<uint4*>(p);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
+__device__ __forceinline__ void store_bucket(BucketRec* p, const Xyzz& a) {
+  uint32_t w[36];
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    w[i] = a.x.l[i];
+    w[NL + i] = a.y.l[i];
+    w[2 * NL + i] = a.zz.l[i];
+    w[3 * NL + i] = a.zzz.l[i];
+  }
+  store_bucket_words(p, w);
+}
+
+__device__ __forceinline__ void store_bucket(BucketRec* p, const Jac& a) {
+  uint32_t w[36];
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    w[i] = a.x.l[i];
+    w[NL + i] = a.y.l[i];
+    w[2 * NL + i] = a.z.l[i];
+  }
+#pragma unroll
+  for (int i = 3 * NL; i < 35; ++i) w[i] = 0;
+  w[35] = kJacMark;
+  store_bucket_words(p, w);
+}
+
+// The bucket as a Jacobian point (XYZZ sums converted: 4 products).
+template <class F>
+__device__ __forceinline__ Jac load_bucket(const F& f, const BucketRec* p) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  uint32_t w[36];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const uint4 v = q[i];
+    w[4 * i] = v.x, w[4 * i + 1] = v.y, w[4 * i + 2] = v.z, w[4 * i + 3] = v.w;
+  }
+  Xyzz a;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    a.x.l[i] = w[i];
+    a.y.l[i] = w[NL + i];
+    a.zz.l[i] = w[2 * NL + i];
+    a.zzz.l[i] = w[3 * NL + i];
+  }
+  if (w[35] & kJacMark) {
+    Jac j;
+    j.x = a.x;
+    j.y = a.y;
+    j.z = a.zz;
+    return j;
+  }
+  return xyzz_to_jac(f, a);
 }
 
 // ---- scalars and digits -----------------------------------------------------
@@ -330,7 +403,7 @@ constexpr uint32_t kValid = 1, kSingle = 2;
 
 // Merges the summary at *l with the one at *r (the range right after it), writing every run that
 // becomes bounded on both sides. At most one point addition.
-__device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, JacRec* __restrict__ buckets) {
+__device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, BucketRec* __restrict__ buckets) {
   const FqIlp f;
   const uint32_t lf = l->flags, rf = r->flags;
   if (!(rf & kValid)) return;
@@ -353,15 +426,15 @@ __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, J
     } else if (rs) {
       store_jac_rec(&l->lsum, joined);
     } else {
-      store_jac_rec(buckets + lk, joined);  // bounded by both ranges' interiors
+      store_bucket(buckets + lk, joined);  // bounded by both ranges' interiors
       l->lk = r->lk;
       l->lsum = r->lsum;
     }
     return;
   }
   // different keys meet at the boundary
-  if (!ls) buckets[l->lk] = l->lsum;  // l's last run is now closed on both sides
-  if (!rs) buckets[r->fk] = r->fsum;  // r's first run likewise
+  if (!ls) store_bucket(buckets + l->lk, load_jac_rec(&l->lsum));  // l's last run is now closed on both sides
+  if (!rs) store_bucket(buckets + r->fk, load_jac_rec(&r->fsum));  // r's first run likewise
   if (ls) {
     // l's single run stays open on the left
   }
@@ -372,7 +445,7 @@ __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, J
 
 // Tree over the N summaries of a workgroup in LDS; the result is left in sh[0].
 template <int N>
-__device__ __forceinline__ void merge_tree(RunSummary* sh, JacRec* __restrict__ buckets) {
+__device__ __forceinline__ void merge_tree(RunSummary* sh, BucketRec* __restrict__ buckets) {
   for (int d = 1; d < N; d <<= 1) {
     __syncthreads();
     if ((threadIdx.x & (2 * d - 1)) == 0) merge_runs(&sh[threadIdx.x], &sh[threadIdx.x + d], buckets);
@@ -381,18 +454,18 @@ __device__ __forceinline__ void merge_tree(RunSummary* sh, JacRec* __restrict__ 
 }
 
 // Writes the open runs of the final summary (they are bounded by the ends of the array).
-__device__ __forceinline__ void close_runs(const RunSummary* sm, JacRec* __restrict__ buckets) {
+__device__ __forceinline__ void close_runs(const RunSummary* sm, BucketRec* __restrict__ buckets) {
   if (!(sm->flags & kValid)) return;
-  buckets[sm->fk] = sm->fsum;
-  if (!(sm->flags & kSingle)) buckets[sm->lk] = sm->lsum;
+  store_bucket(buckets + sm->fk, load_jac_rec(&sm->fsum));
+  if (!(sm->flags & kSingle)) store_bucket(buckets + sm->lk, load_jac_rec(&sm->lsum));
 }
 
 constexpr int kAccThreads = 64;      // one wave per accumulation workgroup: its tree needs 15 KB of LDS
 constexpr int kCombineThreads = 256;
 
 // Level 0, a persistent grid: the m sorted entries are split evenly, ceil(m / threads) consecutive
-// entries per thread, added in XYZZ coordinates (madd-2008-s), and each workgroup merges its
-// threads' run summaries; sums leave as Jacobian points.
+// entries per thread; complete runs leave as bucket records in the accumulator's coordinates, the
+// range's first and last runs as Jacobian run summaries, which each workgroup then merges.
 // The running bucket sum of the accumulation, in Jacobian (madd-2007-bl) or XYZZ (madd-2008-s)
 // coordinates; F picks the product schedule (serial columns, or FqIlp's independent partial sums).
 template <class F, bool kXyzz>
@@ -404,6 +477,7 @@ struct BucketSum<F, false> {
   __device__ __forceinline__ void reset(const F& f) { a = jac_identity(f); }
   __device__ __forceinline__ void add(const F& f, const Aff& q) { a = jac_madd(f, a, q); }
   __device__ __forceinline__ Jac jac(const F&) const { return a; }
+  __device__ __forceinline__ void store(BucketRec* p) const { store_bucket(p, a); }
 };
 
 template <class F>
@@ -412,12 +486,13 @@ struct BucketSum<F, true> {
   __device__ __forceinline__ void reset(const F& f) { a = xyzz_identity(f); }
   __device__ __forceinline__ void add(const F& f, const Aff& q) { a = xyzz_madd(f, a, q); }
   __device__ __forceinline__ Jac jac(const F& f) const { return xyzz_to_jac(f, a); }
+  __device__ __forceinline__ void store(BucketRec* p) const { store_bucket(p, a); }
 };
 
 template <class F, bool kXyzz>
 __global__ void __launch_bounds__(kAccThreads)
 k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_dev,
-             const AffRec* __restrict__ pts, JacRec* __restrict__ buckets, RunSummary* __restrict__ out, int last_level) {
+             const AffRec* __restrict__ pts, BucketRec* __restrict__ buckets, RunSummary* __restrict__ out, int last_level) {
   __shared__ RunSummary sh[kAccThreads];
   const int64_t threads = (int64_t)gridDim.x * kAccThreads;
   const int64_t t = (int64_t)blockIdx.x * kAccThreads + threadIdx.x;
@@ -430,8 +505,9 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
   me.fk = me.lk = 0;
   if (s < m) {
     const int64_t e = min(s + k, m), last = e - 1;
-    BucketSum<F, kXyzz> acc;
+    BucketSum<F, kXyzz> acc, head;
     acc.reset(f);
+    head.reset(f);
     // software pipeline: the point of entry x and the record of entry x + 1 are in flight while
     // entry x - 1 is added (indices past the range are clamped to its last entry, and unused)
     uint2 rec = sorted[s];
@@ -440,27 +516,17 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
     uint32_t cur = rec.x;
     bool first = true;
     me.fk = cur;
-    me.flags = kValid | kSingle;
-    for (int64_t x = s; x <= e; ++x) {
+    for (int64_t x = s; x < e; ++x) {
       const uint2 rec2 = sorted[min(x + 2, last)];
       const AffRaw pt1 = load_aff_raw(pts + (rec1.y & 0x7fffffffu));
-      const uint32_t key = x < e ? rec.x : 0xffffffffu;
-      if (key != cur) {
-        const bool is_last = x == e;
-        if (first && is_last) {
-          store_jac_rec(&me.fsum, acc.jac(f));
-        } else if (first) {
-          store_jac_rec(&me.fsum, acc.jac(f));
-          me.flags = kValid;
-        } else if (is_last) {
-          me.lk = cur;
-          store_jac_rec(&me.lsum, acc.jac(f));
-        } else {
-          store_jac_rec(buckets + cur, acc.jac(f));  // interior run: complete
-        }
-        if (is_last) break;
+      if (rec.x != cur) {
+        // a run ends: the range's first run stays in registers; later ones are complete and
+        // leave as they are (no conversion inside the loop, where some lane ends a run in most
+        // iterations)
+        if (first) head = acc;
+        else acc.store(buckets + cur);
         first = false;
-        cur = key;
+        cur = rec.x;
         acc.reset(f);
       }
       Aff q = unpack_aff(pt);
@@ -469,6 +535,15 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
       rec = rec1;
       rec1 = rec2;
       pt = pt1;
+    }
+    if (first) {
+      store_jac_rec(&me.fsum, acc.jac(f));
+      me.flags = kValid | kSingle;
+    } else {
+      store_jac_rec(&me.fsum, head.jac(f));
+      store_jac_rec(&me.lsum, acc.jac(f));
+      me.lk = cur;
+      me.flags = kValid;
     }
   }
   sh[threadIdx.x] = me;
@@ -481,7 +556,7 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
 
 // Levels >= 1: the `count` summaries of the previous level, 256 per workgroup, merged by the same tree.
 __global__ void __launch_bounds__(kCombineThreads) k_combine(const RunSummary* __restrict__ in, int64_t count,
-                                                             JacRec* __restrict__ buckets, RunSummary* __restrict__ out,
+                                                             BucketRec* __restrict__ buckets, RunSummary* __restrict__ out,
                                                              int last_level) {
   __shared__ RunSummary sh[kCombineThreads];
   const int64_t t = (int64_t)blockIdx.x * kCombineThreads + threadIdx.x;
@@ -500,7 +575,7 @@ __global__ void __launch_bounds__(kCombineThreads) k_combine(const RunSummary* _
 // Leaf of the bucket reduction: segment s of a window covers buckets [sL, sL + L)
 // (bucket b holds the digit value b + 1): T = sum (i + 1) B_{sL+i}, R = sum B_{sL+i}.
 __global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ jobs, int njobs, int64_t segments,
-                                                     const JacRec* __restrict__ buckets, JacRec* __restrict__ tot,
+                                                     const BucketRec* __restrict__ buckets, JacRec* __restrict__ tot,
                                                      JacRec* __restrict__ run) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= segments) return;
@@ -508,11 +583,11 @@ __global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ 
   const int64_t local = g - job.seg_base;
   const int seg_len = 1 << job.log_seg;
   const int64_t win = local >> job.log_segs, s = local & ((1LL << job.log_segs) - 1);
-  const JacRec* b = buckets + job.key_base + (win << (job.c - 1)) + s * seg_len;
+  const BucketRec* b = buckets + job.key_base + (win << (job.c - 1)) + s * seg_len;
   const FqIlp f;
   Jac r = jac_identity(f), t = jac_identity(f);
   for (int i = seg_len - 1; i >= 0; --i) {
-    r = jac_add(f, r, load_jac_rec(b + i));
+    r = jac_add(f, r, load_bucket(f, b + i));
     t = jac_add(f, t, r);
   }
   store_jac_rec(tot + g, t);
@@ -641,7 +716,7 @@ static int g_glv = 1;
 // low-latency product schedule, bit 1 XYZZ coordinates.
 static int g_acc_variant = 0;
 
-using AccKernel = void (*)(const uint2*, const uint32_t*, const AffRec*, JacRec*, RunSummary*, int);
+using AccKernel = void (*)(const uint2*, const uint32_t*, const AffRec*, BucketRec*, RunSummary*, int);
 
 static AccKernel acc_kernel(int variant) {
   switch (variant & 3) {
@@ -688,7 +763,7 @@ struct MsmPlan {
   MsmJob* d_jobs = nullptr;
   uint32_t *counts = nullptr, *totals = nullptr, *starts = nullptr, *m_dev = nullptr, *scan_ws = nullptr;
   uint2* sorted = nullptr;
-  JacRec* buckets = nullptr;
+  BucketRec* buckets = nullptr;
   RunSummary* runs[2] = {nullptr, nullptr};
   JacRec *tot = nullptr, *run = nullptr, *parts_buf = nullptr, *rows_buf = nullptr, *wsum = nullptr;
 };
@@ -792,7 +867,7 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   L->m = take(16);
   L->sorted = take(8 * e1);
   L->scan = take(4 * scan_workspace_words(keys));
-  L->buckets = take(sizeof(JacRec) * (size_t)keys);
+  L->buckets = take(sizeof(BucketRec) * (size_t)keys);
   L->r0 = take(sizeof(RunSummary) * (size_t)max_slots);
   L->r1 = take(sizeof(RunSummary) * (size_t)max_slots);
   L->tot = take(sizeof(JacRec) * (size_t)segs);
@@ -827,7 +902,7 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   plan->m_dev = reinterpret_cast<uint32_t*>(base + L.m);
   plan->sorted = reinterpret_cast<uint2*>(base + L.sorted);
   plan->scan_ws = reinterpret_cast<uint32_t*>(base + L.scan);
-  plan->buckets = reinterpret_cast<JacRec*>(base + L.buckets);
+  plan->buckets = reinterpret_cast<BucketRec*>(base + L.buckets);
   plan->runs[0] = reinterpret_cast<RunSummary*>(base + L.r0);
   plan->runs[1] = reinterpret_cast<RunSummary*>(base + L.r1);
   plan->tot = reinterpret_cast<JacRec*>(base + L.tot);
@@ -859,8 +934,8 @@ void msm_plan_destroy(MsmPlan* plan) {
 int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
   if (plan->njobs == 0) return 0;
   const int nj = plan->njobs;
-  hipLaunchKernelGGL(k_fill_zero<uint32_t>, dim3(grid_for(plan->nkeys * 28, 256)), dim3(256), 0, stream,
-                     reinterpret_cast<uint32_t*>(plan->buckets), plan->nkeys * 28);
+  hipLaunchKernelGGL(k_fill_zero<uint32_t>, dim3(grid_for(plan->nkeys * 36, 256)), dim3(256), 0, stream,
+                     reinterpret_cast<uint32_t*>(plan->buckets), plan->nkeys * 36);
   ZK_LAUNCH_CHECK("msm clear");
   {
     Prof prof("msm_sort", stream);
