@@ -76,60 +76,15 @@ __device__ __forceinline__ Fe lane_pick(int lane, const Fe& a0, const Fe& a1, co
 // ---- lane-parallel linear combinations ---------------------------------------------------
 // Between the product levels, lane k forms the combination its next operand (or output) needs,
 // sum_j c_j(k) V_j + K(k) p over values V_j that every lane holds, with small per-lane integer
-// coefficients, in nine signed 64-bit limb accumulators (one v_mad_i64_i32 per term and limb).
-// Values stay lazily reduced: product operands only need normalized limbs and a bound (a product
-// of operands below 13p stays below 2p), and results are brought below 2p by subtracting a
-// quotient estimated from the top limb. Every coordinate is the same residue as in zk_curve.hpp's
-// formulas; only the representative differs, below 2p, until a caller canonicalizes it.
-
-struct Acc64 {
-  int64_t l[NL];
-};
+// coefficients (Acc64 and lc_* in zk_field.hpp). Values stay lazily reduced: product operands only
+// need normalized limbs and a bound (a product of operands below 13p stays below 2p), and results
+// are brought below 2p by subtracting a quotient estimated from the top limb. Every coordinate is
+// the same residue as in zk_curve.hpp's formulas; only the representative differs, below 2p, until
+// a caller canonicalizes it.
 
 __device__ __forceinline__ int lane_coef(int lane, int c0, int c1) { return lane == 0 ? c0 : c1; }
 __device__ __forceinline__ int lane_coef(int lane, int c0, int c1, int c2) {
   return lane == 0 ? c0 : (lane == 1 ? c1 : c2);
-}
-
-// K p, the offset that keeps a combination non-negative.
-template <class F>
-__device__ __forceinline__ Acc64 lc_start(const F&, int k) {
-  Acc64 a;
-#pragma unroll
-  for (int i = 0; i < NL; ++i) a.l[i] = (int64_t)k * (int64_t)F::p[i];
-  return a;
-}
-
-// a += c v for v with limbs below 2^31.
-__device__ __forceinline__ void lc_add(Acc64& a, int c, const Fe& v) {
-#pragma unroll
-  for (int i = 0; i < NL; ++i) a.l[i] += (int64_t)c * (int64_t)(int32_t)v.l[i];
-}
-
-// Limbs 0..7 into [0, 2^29) with arithmetic carries; the top limb takes the rest (non-negative
-// for a non-negative value).
-__device__ __forceinline__ Fe lc_carry(Acc64 a) {
-  Fe r;
-#pragma unroll
-  for (int i = 0; i < NL - 1; ++i) {
-    a.l[i + 1] += a.l[i] >> LB;
-    r.l[i] = (uint32_t)a.l[i] & LMASK;
-  }
-  r.l[NL - 1] = (uint32_t)a.l[NL - 1];
-  return r;
-}
-
-// The value V (0 <= V < 2^256, limbs below 2^36 in magnitude) less q p, q = floor(t / (p_8 + 1))
-// for t = the top limb less a margin for the uncarried lower limbs (they move V / 2^232 by less
-// than 2^8): q p <= V, and V - q p < p (1 + 2^13 / p_8) < 2p for the BN254 moduli (p_8 > 2^21).
-template <class F>
-__device__ __forceinline__ Fe lc_finish(const F&, Acc64 a) {
-  static_assert(F::p[NL - 1] > (1u << 21), "the quotient estimate needs a full-size top limb");
-  const int64_t t = a.l[NL - 1] - 4096;
-  const int32_t q = t > 0 ? (int32_t)((uint32_t)t / (F::p[NL - 1] + 1u)) : 0;
-#pragma unroll
-  for (int i = 0; i < NL; ++i) a.l[i] -= (int64_t)q * (int64_t)F::p[i];
-  return lc_carry(a);
 }
 
 // x + y limb by limb, no carries: for x, y with limbs below 2^29 the sum's limbs stay below 2^30.

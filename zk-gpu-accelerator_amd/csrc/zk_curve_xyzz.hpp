@@ -5,7 +5,8 @@
 // the Explicit-Formulas Database) instead of the 7 + 4 of madd-2007-bl in Jacobian
 // coordinates, with half the additions. Buckets leave the accumulation as Jacobian points:
 // with ZZ = Z^2 and ZZZ = Z^3, (X ZZ^2, Y ZZZ^2, ZZZ) is (x Z^6, y Z^9, Z^3), the Jacobian
-// point with Z' = Z^3. Coordinates are in the 261-form and canonical, as in zk_curve.hpp.
+// point with Z' = Z^3. Coordinates are in the 261-form; the addition keeps them below 2p (lazily
+// reduced), and the conversion to Jacobian coordinates returns canonical ones.
 #pragma once
 
 #include "zk_curve.hpp"
@@ -48,6 +49,9 @@ __device__ __forceinline__ Xyzz xyzz_dbl_affine(const F& f, const Aff& q) {
 // P + (x2, y2) for a finite affine point, madd-2008-s: U2 = x2 ZZ1, S2 = y2 ZZZ1, P = U2 - X1,
 // R = S2 - Y1, PP = P^2, PPP = P PP, Q = X1 PP, X3 = R^2 - PPP - 2Q, Y3 = R (Q - X3) - Y1 PPP,
 // ZZ3 = ZZ1 PP, ZZZ3 = ZZZ1 PPP. P = 0 doubles (R = 0) or gives the identity.
+// Lazily reduced: P's coordinates below 2p (the identity has ZZ exactly 0, and a finite point's ZZ
+// is never 0 mod p), products without their final subtraction, R and Q - X3 as unreduced operands,
+// X3 from one signed combination; the result's coordinates are below 2p.
 template <class F>
 __device__ __forceinline__ Xyzz xyzz_madd(const F& f, const Xyzz& p, const Aff& q) {
   if (fe_is_zero(p.zz)) {
@@ -58,19 +62,24 @@ __device__ __forceinline__ Xyzz xyzz_madd(const F& f, const Xyzz& p, const Aff& 
     r.zzz = r.zz;
     return r;
   }
-  const Fe u2 = mont(f, q.x, p.zz);
-  const Fe s2 = mont(f, q.y, p.zzz);
-  const Fe pp0 = fsub(f, u2, p.x);
-  const Fe rr = fsub(f, s2, p.y);
-  if (fe_is_zero(pp0)) return fe_is_zero(rr) ? xyzz_dbl_affine(f, q) : xyzz_identity(f);
-  const Fe pp = mont_sqr(f, pp0);
-  const Fe ppp = mont(f, pp0, pp);
-  const Fe qq = mont(f, p.x, pp);
+  const Fe u2 = mont_raw(f, q.x, p.zz);
+  const Fe s2 = mont_raw(f, q.y, p.zzz);
+  const Fe pp0 = fsub_lazy(f, u2, p.x);  // [0, 2p)
+  if (fe_is_zero(pp0) || fe_eq(pp0, fe_const<F>(f.p)))
+    return fe_eq(canonical(f, s2), canonical(f, p.y)) ? xyzz_dbl_affine(f, q) : xyzz_identity(f);
+  const Fe rr = fsub_norm(f, s2, p.y);  // R + 2p, below 4p
+  const Fe pp = sqr_raw(f, pp0);
+  const Fe ppp = mont_raw(f, pp0, pp);
+  const Fe qq = mont_raw(f, p.x, pp);
+  Acc64 xa = lc_start(f, 6);  // R^2 - PPP - 2Q + 6p in (0, 8p)
+  lc_add(xa, 1, sqr_raw(f, rr));
+  lc_add(xa, -1, ppp);
+  lc_add(xa, -2, qq);
   Xyzz r;
-  r.x = fsub(f, fsub(f, mont_sqr(f, rr), ppp), fdbl(f, qq));
-  r.y = fsub(f, mont(f, rr, fsub(f, qq, r.x)), mont(f, p.y, ppp));
-  r.zz = mont(f, p.zz, pp);
-  r.zzz = mont(f, p.zzz, ppp);
+  r.x = lc_finish(f, xa);
+  r.y = fsub_lazy(f, mont_raw(f, rr, fsub_for_mul(f, qq, r.x)), mont_raw(f, p.y, ppp));
+  r.zz = mont_raw(f, p.zz, pp);
+  r.zzz = mont_raw(f, p.zzz, ppp);
   return r;
 }
 
@@ -81,7 +90,7 @@ __device__ __forceinline__ Jac xyzz_to_jac(const F& f, const Xyzz& p) {
   Jac r;
   r.x = mont(f, p.x, mont_sqr(f, p.zz));
   r.y = mont(f, p.y, mont_sqr(f, p.zzz));
-  r.z = p.zzz;
+  r.z = canonical(f, p.zzz);
   return r;
 }
 

@@ -328,6 +328,28 @@ __device__ __forceinline__ Fe fsub_for_mul(const F& f, const Fe& a, const Fe& b)
   return d;
 }
 
+// a - b + 2p in (0, 4p) with normalized limbs (signed carries), for a, b < 2p: an operand that
+// may be squared (both operands of a product then have limbs below 2^29).
+template <class F>
+__device__ __forceinline__ Fe fsub_norm(const F& f, const Fe& a, const Fe& b) {
+  Fe d;
+  int32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const int32_t t = (int32_t)(a.l[i] + f.p2[i]) - (int32_t)b.l[i] + c;  // |t| < 2^31
+    d.l[i] = (uint32_t)t & LMASK;
+    c = t >> LB;
+  }
+  return d;
+}
+
+// x^2 * 2^-261 below 2p with the field's product schedule (the squaring shortcut for serial columns).
+template <class F>
+__device__ __forceinline__ Fe sqr_raw(const F& f, const Fe& a) {
+  if constexpr (UsesIlp<F>::value) return mont_raw_ilp(f, a, a);
+  else return mont_sqr_raw(f, a);
+}
+
 // a - b in [0, 2p) for a, b < 2p.
 template <class F>
 __device__ __forceinline__ Fe fsub_lazy(const F& f, const Fe& a, const Fe& b) {
@@ -347,6 +369,57 @@ __device__ __forceinline__ Fe fsub_lazy(const F& f, const Fe& a, const Fe& b) {
     c = t >> LB;
   }
   return d;
+}
+
+// ---- small signed combinations (chains of additions and products with lazy reduction) ----
+// sum_j c_j V_j + K p with small integer coefficients, in nine signed 64-bit limb accumulators (one
+// v_mad_i64_i32 per term and limb), then either normalized (lc_carry: a product operand, only a
+// bound needed) or brought below 2p (lc_finish: a quotient estimated from the top limb, for the
+// BN254 moduli).
+
+struct Acc64 {
+  int64_t l[NL];
+};
+
+// K p, the offset that keeps a combination non-negative.
+template <class F>
+__device__ __forceinline__ Acc64 lc_start(const F&, int k) {
+  Acc64 a;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) a.l[i] = (int64_t)k * (int64_t)F::p[i];
+  return a;
+}
+
+// a += c v for v with limbs below 2^31.
+__device__ __forceinline__ void lc_add(Acc64& a, int c, const Fe& v) {
+#pragma unroll
+  for (int i = 0; i < NL; ++i) a.l[i] += (int64_t)c * (int64_t)(int32_t)v.l[i];
+}
+
+// Limbs 0..7 into [0, 2^29) with arithmetic carries; the top limb takes the rest (non-negative
+// for a non-negative value).
+__device__ __forceinline__ Fe lc_carry(Acc64 a) {
+  Fe r;
+#pragma unroll
+  for (int i = 0; i < NL - 1; ++i) {
+    a.l[i + 1] += a.l[i] >> LB;
+    r.l[i] = (uint32_t)a.l[i] & LMASK;
+  }
+  r.l[NL - 1] = (uint32_t)a.l[NL - 1];
+  return r;
+}
+
+// The value V (0 <= V < 2^256, limbs below 2^36 in magnitude) less q p, q = floor(t / (p_8 + 1))
+// for t = the top limb less a margin for the uncarried lower limbs (they move V / 2^232 by less
+// than 2^8): q p <= V, and V - q p < p (1 + 2^13 / p_8) < 2p for the BN254 moduli (p_8 > 2^21).
+template <class F>
+__device__ __forceinline__ Fe lc_finish(const F&, Acc64 a) {
+  static_assert(F::p[NL - 1] > (1u << 21), "the quotient estimate needs a full-size top limb");
+  const int64_t t = a.l[NL - 1] - 4096;
+  const int32_t q = t > 0 ? (int32_t)((uint32_t)t / (F::p[NL - 1] + 1u)) : 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) a.l[i] -= (int64_t)q * (int64_t)F::p[i];
+  return lc_carry(a);
 }
 
 template <class F>
