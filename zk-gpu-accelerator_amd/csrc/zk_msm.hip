@@ -402,20 +402,27 @@ struct alignas(16) RunSummary {
 constexpr uint32_t kValid = 1, kSingle = 2;
 
 // Merges the summary at *l with the one at *r (the range right after it), writing every run that
-// becomes bounded on both sides. At most one point addition.
+// becomes bounded on both sides. At most one point addition: by one thread, or (kRow) by the 16
+// lanes of a row with the wave-parallel formulas, whose first lane then does the writes.
+template <bool kRow>
 __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, BucketRec* __restrict__ buckets) {
   const FqIlp f;
+  const bool lead = !kRow || (threadIdx.x & 15) == 0;
   const uint32_t lf = l->flags, rf = r->flags;
   if (!(rf & kValid)) return;
   if (!(lf & kValid)) {
-    *l = *r;
+    if (lead) *l = *r;
     return;
   }
   const bool ls = lf & kSingle, rs = rf & kSingle;
   const uint32_t lk = ls ? l->fk : l->lk;
   if (lk == r->fk) {
     // the run crossing the boundary
-    Jac joined = jac_add(f, load_jac_rec(ls ? &l->fsum : &l->lsum), load_jac_rec(&r->fsum));
+    const Jac a = load_jac_rec(ls ? &l->fsum : &l->lsum), b = load_jac_rec(&r->fsum);
+    Jac joined;
+    if constexpr (kRow) joined = jac_canonical(f, jac_add_wave(f, a, b));
+    else joined = jac_add(f, a, b);
+    if (!lead) return;
     if (ls && rs) {
       store_jac_rec(&l->fsum, joined);
     } else if (ls) {
@@ -432,23 +439,32 @@ __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, B
     }
     return;
   }
+  if (!lead) return;
   // different keys meet at the boundary
   if (!ls) store_bucket(buckets + l->lk, load_jac_rec(&l->lsum));  // l's last run is now closed on both sides
   if (!rs) store_bucket(buckets + r->fk, load_jac_rec(&r->fsum));  // r's first run likewise
-  if (ls) {
-    // l's single run stays open on the left
-  }
+  // (a single run of l stays open on the left)
   l->lk = rs ? r->fk : r->lk;
   l->lsum = rs ? r->fsum : r->lsum;
   l->flags = kValid;
 }
 
-// Tree over the N summaries of a workgroup in LDS; the result is left in sh[0].
+// Tree over the N summaries of a workgroup in LDS; the result is left in sh[0]. Levels with more
+// merges than the workgroup has 16-lane rows run one merge per thread; the upper levels, where a
+// thread-serial addition would leave the level's latency at 16 products in a row, run one merge
+// per row (5 product levels).
 template <int N>
 __device__ __forceinline__ void merge_tree(RunSummary* sh, BucketRec* __restrict__ buckets) {
+  constexpr int kRows = N / 16;
   for (int d = 1; d < N; d <<= 1) {
     __syncthreads();
-    if ((threadIdx.x & (2 * d - 1)) == 0) merge_runs(&sh[threadIdx.x], &sh[threadIdx.x + d], buckets);
+    const int merges = N / (2 * d);
+    if (merges > kRows) {
+      if ((threadIdx.x & (2 * d - 1)) == 0) merge_runs<false>(&sh[threadIdx.x], &sh[threadIdx.x + d], buckets);
+    } else {
+      const int row = (int)threadIdx.x >> 4;
+      if (row < merges) merge_runs<true>(&sh[2 * d * row], &sh[2 * d * row + d], buckets);
+    }
   }
   __syncthreads();
 }
