@@ -449,18 +449,19 @@ __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, B
   l->flags = kValid;
 }
 
-// Tree over the N summaries of a workgroup in LDS; the result is left in sh[0]. Levels with more
-// merges than the workgroup has 16-lane rows run one merge per thread; the upper levels, where a
-// thread-serial addition would leave the level's latency at 16 products in a row, run one merge
-// per row (5 product levels).
-template <int N>
-__device__ __forceinline__ void merge_tree(RunSummary* sh, BucketRec* __restrict__ buckets) {
-  constexpr int kRows = N / 16;
-  for (int d = 1; d < N; d <<= 1) {
+// Tree over the first `width` (a power of two) summaries of a workgroup in LDS; the result is left
+// in sh[0]. Levels with more merges than the workgroup has 16-lane rows run one merge per thread;
+// the upper levels, where a thread-serial addition would leave the level's latency at 16 products
+// in a row, run one merge per row (5 product levels).
+__device__ __forceinline__ void merge_tree(RunSummary* sh, int width, BucketRec* __restrict__ buckets) {
+  const int rows = (int)blockDim.x >> 4;
+  for (int d = 1; d < width; d <<= 1) {
     __syncthreads();
-    const int merges = N / (2 * d);
-    if (merges > kRows) {
-      if ((threadIdx.x & (2 * d - 1)) == 0) merge_runs<false>(&sh[threadIdx.x], &sh[threadIdx.x + d], buckets);
+    const int merges = width / (2 * d);
+    if (merges > rows) {
+      // merge i on thread i: the active lanes fill the leading waves, and the rest skip the addition
+      const int i = (int)threadIdx.x;
+      if (i < merges) merge_runs<false>(&sh[2 * d * i], &sh[2 * d * i + d], buckets);
     } else {
       const int row = (int)threadIdx.x >> 4;
       if (row < merges) merge_runs<true>(&sh[2 * d * row], &sh[2 * d * row + d], buckets);
@@ -476,12 +477,13 @@ __device__ __forceinline__ void close_runs(const RunSummary* sm, BucketRec* __re
   if (!(sm->flags & kSingle)) store_bucket(buckets + sm->lk, load_jac_rec(&sm->lsum));
 }
 
-constexpr int kAccThreads = 64;      // one wave per accumulation workgroup: its tree needs 15 KB of LDS
+constexpr int kAccThreads = 64;      // one wave per accumulation workgroup
 constexpr int kCombineThreads = 256;
 
 // Level 0, a persistent grid: the m sorted entries are split evenly, ceil(m / threads) consecutive
 // entries per thread; complete runs leave as bucket records in the accumulator's coordinates, the
-// range's first and last runs as Jacobian run summaries, which each workgroup then merges.
+// range's first and last runs as one Jacobian run summary per thread, merged by k_combine (no tree
+// here: its point additions would raise the kernel's registers past three waves per SIMD).
 // The running bucket sum of the accumulation, in Jacobian (madd-2007-bl) or XYZZ (madd-2008-s)
 // coordinates; F picks the product schedule (serial columns, or FqIlp's independent partial sums).
 template <class F, bool kXyzz>
@@ -508,8 +510,7 @@ struct BucketSum<F, true> {
 template <class F, bool kXyzz>
 __global__ void __launch_bounds__(kAccThreads)
 k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_dev,
-             const AffRec* __restrict__ pts, BucketRec* __restrict__ buckets, RunSummary* __restrict__ out, int last_level) {
-  __shared__ RunSummary sh[kAccThreads];
+             const AffRec* __restrict__ pts, BucketRec* __restrict__ buckets, RunSummary* __restrict__ out) {
   const int64_t threads = (int64_t)gridDim.x * kAccThreads;
   const int64_t t = (int64_t)blockIdx.x * kAccThreads + threadIdx.x;
   const int64_t m = *m_dev;
@@ -521,9 +522,8 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
   me.fk = me.lk = 0;
   if (s < m) {
     const int64_t e = min(s + k, m), last = e - 1;
-    BucketSum<F, kXyzz> acc, head;
+    BucketSum<F, kXyzz> acc;
     acc.reset(f);
-    head.reset(f);
     // software pipeline: the point of entry x and the record of entry x + 1 are in flight while
     // entry x - 1 is added (indices past the range are clamped to its last entry, and unused)
     uint2 rec = sorted[s];
@@ -536,11 +536,11 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
       const uint2 rec2 = sorted[min(x + 2, last)];
       const AffRaw pt1 = load_aff_raw(pts + (rec1.y & 0x7fffffffu));
       if (rec.x != cur) {
-        // a run ends: the range's first run stays in registers; later ones are complete and
-        // leave as they are (no conversion inside the loop, where some lane ends a run in most
-        // iterations)
-        if (first) head = acc;
-        else acc.store(buckets + cur);
+        // a run ends and leaves as it is (no conversion inside the loop, where some lane ends a
+        // run in most iterations). The range's first run may continue in the previous range; it
+        // is parked in its own bucket, which nothing else writes before the run summaries are
+        // merged, and taken back below.
+        acc.store(buckets + cur);
         first = false;
         cur = rec.x;
         acc.reset(f);
@@ -556,21 +556,16 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
       store_jac_rec(&me.fsum, acc.jac(f));
       me.flags = kValid | kSingle;
     } else {
-      store_jac_rec(&me.fsum, head.jac(f));
       store_jac_rec(&me.lsum, acc.jac(f));
+      store_jac_rec(&me.fsum, load_bucket(f, buckets + me.fk));
       me.lk = cur;
       me.flags = kValid;
     }
   }
-  sh[threadIdx.x] = me;
-  merge_tree<kAccThreads>(sh, buckets);
-  if (threadIdx.x == 0) {
-    if (last_level) close_runs(&sh[0], buckets);
-    else out[blockIdx.x] = sh[0];
-  }
+  out[t] = me;
 }
 
-// Levels >= 1: the `count` summaries of the previous level, 256 per workgroup, merged by the same tree.
+// Levels >= 1: the `count` summaries of the previous level, 256 per workgroup, merged by a tree.
 __global__ void __launch_bounds__(kCombineThreads) k_combine(const RunSummary* __restrict__ in, int64_t count,
                                                              BucketRec* __restrict__ buckets, RunSummary* __restrict__ out,
                                                              int last_level) {
@@ -581,7 +576,10 @@ __global__ void __launch_bounds__(kCombineThreads) k_combine(const RunSummary* _
   } else {
     sh[threadIdx.x].flags = 0;
   }
-  merge_tree<kCombineThreads>(sh, buckets);
+  const int64_t valid = min((int64_t)kCombineThreads, count - (int64_t)blockIdx.x * kCombineThreads);
+  int width = 1;
+  while (width < valid) width <<= 1;
+  merge_tree(sh, width, buckets);
   if (threadIdx.x == 0) {
     if (last_level) close_runs(&sh[0], buckets);
     else out[blockIdx.x] = sh[0];
@@ -732,7 +730,7 @@ static int g_glv = 1;
 // low-latency product schedule, bit 1 XYZZ coordinates.
 static int g_acc_variant = 0;
 
-using AccKernel = void (*)(const uint2*, const uint32_t*, const AffRec*, BucketRec*, RunSummary*, int);
+using AccKernel = void (*)(const uint2*, const uint32_t*, const AffRec*, BucketRec*, RunSummary*);
 
 static AccKernel acc_kernel(int variant) {
   switch (variant & 3) {
@@ -767,8 +765,9 @@ struct MsmPlan {
   int njobs = 0;
   std::vector<MsmJob> jobs;
   std::vector<int64_t> level_max;
-  int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, segments = 0, sums_rows = 0, parts = 0, windows_total = 0;
-  int64_t sort_blocks = 0;
+  int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, max_slots1 = 0, segments = 0, sums_rows = 0, parts = 0,
+          windows_total = 0;
+  int64_t sort_blocks = 0, acc_blocks = 0;
   size_t bytes = 0;
   void* srs = nullptr;  // the SRS handle the plan reads, if any
   int acc_variant = 0;  // the accumulation kernel (g_acc_variant when planned)
@@ -861,13 +860,16 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   plan->sort_blocks = sort_blocks;
   plan->parts = parts;
   plan->windows_total = wins;
-  // Accumulation levels: level 0 leaves one run summary per workgroup; each further level merges
-  // 256 summaries per workgroup, until a single workgroup remains.
-  plan->level_max.assign(1, grid0);
+  // Accumulation levels: level 0 leaves one run summary per thread; each further level merges 256
+  // summaries per workgroup, until a single workgroup remains. Levels alternate between two buffers.
+  plan->acc_blocks = grid0;
+  plan->level_max.assign(1, grid0 * kAccThreads);
   while (plan->level_max.back() > 1) plan->level_max.push_back((plan->level_max.back() + kCombineThreads - 1) / kCombineThreads);
-  int64_t max_slots = 1;
-  for (int64_t c : plan->level_max) max_slots = std::max(max_slots, c);
-  plan->max_slots = max_slots;
+  int64_t slots[2] = {1, 1};
+  for (size_t l = 0; l < plan->level_max.size(); ++l) slots[l & 1] = std::max(slots[l & 1], plan->level_max[l]);
+  plan->max_slots = slots[0];
+  plan->max_slots1 = slots[1];
+  const int64_t max_slots = slots[0], max_slots1 = slots[1];
   const size_t e1 = (size_t)std::max<int64_t>(entries, 1);
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -885,7 +887,7 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   L->scan = take(4 * scan_workspace_words(keys));
   L->buckets = take(sizeof(BucketRec) * (size_t)keys);
   L->r0 = take(sizeof(RunSummary) * (size_t)max_slots);
-  L->r1 = take(sizeof(RunSummary) * (size_t)max_slots);
+  L->r1 = take(sizeof(RunSummary) * (size_t)max_slots1);
   L->tot = take(sizeof(JacRec) * (size_t)segs);
   L->run = take(sizeof(JacRec) * (size_t)segs);
   L->parts = take(sizeof(JacRec) * (size_t)parts);
@@ -971,8 +973,8 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
   const int levels = (int)plan->level_max.size();
   {
     Prof prof("msm_accumulate", stream);
-    hipLaunchKernelGGL(acc_kernel(plan->acc_variant), dim3((unsigned)plan->level_max[0]), dim3(kAccThreads), 0, stream, plan->sorted,
-                       plan->m_dev, plan->points, plan->buckets, plan->runs[0], levels == 1 ? 1 : 0);
+    hipLaunchKernelGGL(acc_kernel(plan->acc_variant), dim3((unsigned)plan->acc_blocks), dim3(kAccThreads), 0, stream, plan->sorted,
+                       plan->m_dev, plan->points, plan->buckets, plan->runs[0]);
     ZK_LAUNCH_CHECK("msm accumulate");
   }
   {
