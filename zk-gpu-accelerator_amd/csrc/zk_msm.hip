@@ -643,11 +643,21 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
   }
   store_jac_rec(&sh[threadIdx.x], acc);
   __syncthreads();
+  // a tree over the 256 partial sums: one addition per thread while the level has more additions
+  // than the workgroup has 16-lane rows, then one per row (wave-parallel formulas)
   for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) {
-      Jac a = load_jac_rec(&sh[threadIdx.x]);
-      a = jac_add(f, a, load_jac_rec(&sh[threadIdx.x + off]));
-      store_jac_rec(&sh[threadIdx.x], a);
+    if (off > 16) {
+      if ((int)threadIdx.x < off) {
+        Jac a = load_jac_rec(&sh[threadIdx.x]);
+        a = jac_add(f, a, load_jac_rec(&sh[threadIdx.x + off]));
+        store_jac_rec(&sh[threadIdx.x], a);
+      }
+    } else {
+      const int row = (int)threadIdx.x >> 4;
+      if (row < off) {
+        const Jac a = jac_add_wave(f, load_jac_rec(&sh[row]), load_jac_rec(&sh[row + off]));
+        if ((threadIdx.x & 15) == 0) store_jac_rec(&sh[row], off == 1 ? jac_canonical(f, a) : a);
+      }
     }
     __syncthreads();
   }
@@ -667,10 +677,16 @@ __global__ void __launch_bounds__(64) k_row_sums(const MsmJob* __restrict__ jobs
   store_jac_rec(row_out + g, r);
 }
 
-// One wavefront per (job, window): U = sum T_s + 2^log_seg * sum_j 2^j Q_j (Horner over j), with
-// the wave-parallel formulas (the chain is latency-bound).
-__global__ void __launch_bounds__(64) k_window_sums(const MsmJob* __restrict__ jobs, int njobs, int64_t total_windows,
-                                                    const JacRec* __restrict__ rows, JacRec* __restrict__ wsum) {
+// One workgroup of 16 rows per (job, window): U = sum T_s + sum_j 2^(j + log_seg) Q_j. Row r takes
+// the terms r, r + 16, ...: term j < log_segs is Q_j doubled j + log_seg times, the last is sum T_s;
+// the rows' sums then meet in a tree. The doublings of different terms run side by side (a Horner
+// pass over j would chain all of them with the additions), each row with the wave-parallel formulas.
+constexpr int kWinRows = 16;
+
+__global__ void __launch_bounds__(16 * kWinRows) k_window_sums(const MsmJob* __restrict__ jobs, int njobs,
+                                                               int64_t total_windows, const JacRec* __restrict__ rows,
+                                                               JacRec* __restrict__ wsum) {
+  __shared__ JacRec sh[kWinRows];
   const int64_t g = blockIdx.x;
   if (g >= total_windows) return;
   const MsmJob& job = jobs[find_job<&MsmJob::win_base>(jobs, njobs, g)];
@@ -678,14 +694,24 @@ __global__ void __launch_bounds__(64) k_window_sums(const MsmJob* __restrict__ j
   const int per = job.log_segs + 1;
   const JacRec* p = rows + job.sums_base + win * per;
   const FqIlp f;
+  const int row = (int)threadIdx.x >> 4;
   Jac acc = jac_identity(f);
-  for (int j = job.log_segs - 1; j >= 0; --j) {
-    acc = jac_dbl_wave(f, acc);
-    acc = jac_add_wave(f, acc, load_jac_rec(p + j));
+  for (int t = row; t < per; t += kWinRows) {
+    Jac d = load_jac_rec(p + t);
+    const int dbl = t < job.log_segs ? t + job.log_seg : 0;
+    for (int e = 0; e < dbl; ++e) d = jac_dbl_wave(f, d);
+    acc = jac_add_wave(f, acc, d);
   }
-  for (int k = 0; k < job.log_seg; ++k) acc = jac_dbl_wave(f, acc);
-  acc = jac_add_wave(f, acc, load_jac_rec(p + job.log_segs));
-  if (threadIdx.x == 0) store_jac_rec(wsum + g, acc);
+  if ((threadIdx.x & 15) == 0) store_jac_rec(&sh[row], acc);
+  for (int d = 1; d < kWinRows; d <<= 1) {
+    __syncthreads();
+    if (row < kWinRows / (2 * d)) {
+      const Jac a = jac_add_wave(f, load_jac_rec(&sh[2 * d * row]), load_jac_rec(&sh[2 * d * row + d]));
+      if ((threadIdx.x & 15) == 0) store_jac_rec(&sh[2 * d * row], a);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) wsum[g] = sh[0];
 }
 
 // One wavefront per job: Horner over the windows, sum_w 2^(cw) U_w (wave-parallel formulas).
@@ -999,7 +1025,7 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
     hipLaunchKernelGGL(k_row_sums, dim3(grid_for(plan->sums_rows, 64)), dim3(64), 0, stream, plan->d_jobs, nj,
                        plan->sums_rows, plan->parts_buf, plan->rows_buf);
     ZK_LAUNCH_CHECK("msm row sums");
-    hipLaunchKernelGGL(k_window_sums, dim3((unsigned)plan->windows_total), dim3(64), 0, stream, plan->d_jobs, nj,
+    hipLaunchKernelGGL(k_window_sums, dim3((unsigned)plan->windows_total), dim3(16 * kWinRows), 0, stream, plan->d_jobs, nj,
                        plan->windows_total, plan->rows_buf, plan->wsum);
     ZK_LAUNCH_CHECK("msm window sums");
   }
