@@ -32,8 +32,9 @@
 //              conversion inside the loop: some lane of a wave ends a run in most
 //              iterations, so a conversion there would cost the whole wave 4 products
 //              per entry; the reduction converts each bucket once). The runs cut by range
-//              boundaries are merged by a tree of run summaries (workgroup trees in
-//              LDS, then levels of 256), each merge at most one addition. The load
+//              boundaries leave as one run summary per thread and are merged by trees
+//              of 256 summaries per workgroup, level after level, each merge at most
+//              one addition. The load
 //              is balanced whatever the bucket sizes, with no host sync (the
 //              reference splits buckets into chunks level by level and syncs the
 //              host per level, msm_gpu.py:287-333).
@@ -41,10 +42,12 @@
 //              converted to Jacobian coordinates as it is loaded) with
 //              running sums (T = sum (i+1) B, R = sum B); a window's total is
 //              sum_s T_s + L * sum_j 2^j Q_j with Q_j = sum of R_s over segments s
-//              whose index has bit j set, each Q_j a block-wide tree sum.
-//  6. combine  one Horner pass over all bit positions of all windows gives the
-//              Jacobian result on the device (the reference combines windows on
-//              the CPU, msm_gpu.py:449 and msm.py:156-169).
+//              whose index has bit j set, each Q_j (and sum T_s, in two halves)
+//              block-wide tree sums with the same number of additions per block.
+//  6. combine  per window, the terms 2^(j + log L) Q_j are doubled side by side and
+//              summed; a Horner chain over the windows then gives the Jacobian
+//              result on the device, both with wave-parallel formulas (the reference
+//              combines windows on the CPU, msm_gpu.py:449 and msm.py:156-169).
 //
 // The result is a unique group element, so it equals the reference's
 // msm_pippenger for every input; intermediate coordinates never leave the device.
@@ -225,7 +228,6 @@ struct MsmJob {
   int64_t block_base;       // first 256-scalar digit block of this job
   int64_t key_base;         // first bucket key
   int64_t seg_base;         // first reduction segment
-  int64_t sums_base;        // first (window, bit) row of the bucket reduction
   int64_t part_base;        // first (window, bit, chunk) partial of the bucket reduction
   int64_t win_base;         // first per-window sum
   int64_t dig_base;         // first signed digit (window-major: window w at dig_base + w * n)
@@ -608,18 +610,22 @@ __global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ 
   store_jac_rec(run + g, r);
 }
 
-constexpr int kRedPerThread = 8;                      // segments each thread adds serially
+constexpr int kRedPerThread = 4;                      // segments each thread adds serially
 constexpr int kRedChunk = 256 * kRedPerThread;          // segments per workgroup
 
-// One workgroup per (job, window, j, chunk): over segments s of the chunk, j < log_segs sums
-// R_s for s with bit j set (Q_j), j = log_segs sums T_s. 256 threads x 8 segments, then a tree.
+// Rows of a window's reduction: Q_j for j < log_segs (the segments whose index has bit j set,
+// half of them), then the T sum in two halves (segments below and above segs / 2), so that every
+// row has the same number of additions.
+__device__ __forceinline__ int red_rows(const MsmJob& job) { return job.log_segs + 2; }
+
+// One workgroup per (job, window, row, chunk): 256 threads x 4 selected segments, then a tree.
 __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ jobs, int njobs,
                                                      const JacRec* __restrict__ tot, const JacRec* __restrict__ run,
                                                      JacRec* __restrict__ parts) {
   __shared__ JacRec sh[256];
   const MsmJob& job = jobs[find_job<&MsmJob::part_base>(jobs, njobs, blockIdx.x)];
   const int64_t local = (int64_t)blockIdx.x - job.part_base;
-  const int per = job.log_segs + 1;
+  const int per = red_rows(job);
   const int64_t row = local / job.chunks;
   const int64_t chunk = local % job.chunks;
   const int64_t win = row / per;
@@ -627,19 +633,23 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
   const int64_t segs = 1LL << job.log_segs;
   const FqIlp f;
   Jac acc = jac_identity(f);
-  const JacRec* src = (j == job.log_segs ? tot : run) + job.seg_base + win * segs;
-  const int64_t lo = chunk * kRedChunk, hi = min(segs, lo + (int64_t)kRedChunk);
-  if (j == job.log_segs || ((1LL << j) >= kRedChunk && ((lo >> j) & 1))) {
-    // the T row, or a bit that is set across the whole chunk (chunks are 2^11-aligned)
-    for (int64_t s = lo + threadIdx.x; s < hi; s += 256) acc = jac_add(f, acc, load_jac_rec(src + s));
-  } else if ((1LL << j) < kRedChunk) {
-    // only the segments with bit j set: half of [lo, hi), whose ends are multiples of 2^(j+1), so
-    // that neighbouring lanes never skip adds the wavefront executes anyway
-    const int64_t low = (1LL << j) - 1;
-    for (int64_t k = threadIdx.x; k < (hi - lo) >> 1; k += 256) {
-      const int64_t s = lo + ((k >> j) << (j + 1)) + (1LL << j) + (k & low);
-      acc = jac_add(f, acc, load_jac_rec(src + s));
+  const int64_t lo = chunk * kRedChunk;
+  if (j < job.log_segs) {
+    // the k-th segment with bit j set, k in this chunk: whole runs of 2^j segments, so that
+    // neighbouring lanes never skip additions the wavefront executes anyway
+    const JacRec* src = run + job.seg_base + win * segs;
+    const int64_t hi = min(segs >> 1, lo + (int64_t)kRedChunk), low = (1LL << j) - 1;
+    for (int64_t k = lo + threadIdx.x; k < hi; k += 256) {
+      const int64_t sg = ((k >> j) << (j + 1)) + (1LL << j) + (k & low);
+      acc = jac_add(f, acc, load_jac_rec(src + sg));
     }
+  } else {
+    // the T sum over one half of the segments
+    const int64_t first = (segs + 1) >> 1;
+    const int64_t base = j == job.log_segs ? 0 : first, count = j == job.log_segs ? first : segs - first;
+    const JacRec* src = tot + job.seg_base + win * segs + base;
+    const int64_t hi = min(count, lo + (int64_t)kRedChunk);
+    for (int64_t k = lo + threadIdx.x; k < hi; k += 256) acc = jac_add(f, acc, load_jac_rec(src + k));
   }
   store_jac_rec(&sh[threadIdx.x], acc);
   __syncthreads();
@@ -664,40 +674,29 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
   if (threadIdx.x == 0) parts[blockIdx.x] = sh[0];
 }
 
-// One thread per (job, window, row): the row's chunk partials summed.
-__global__ void __launch_bounds__(64) k_row_sums(const MsmJob* __restrict__ jobs, int njobs, int64_t rows,
-                                                 const JacRec* __restrict__ parts, JacRec* __restrict__ row_out) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= rows) return;
-  const MsmJob& job = jobs[find_job<&MsmJob::sums_base>(jobs, njobs, g)];
-  const JacRec* p = parts + job.part_base + (g - job.sums_base) * job.chunks;
-  const FqIlp f;
-  Jac r = load_jac_rec(p);
-  for (int k = 1; k < job.chunks; ++k) r = jac_add(f, r, load_jac_rec(p + k));
-  store_jac_rec(row_out + g, r);
-}
-
 // One workgroup of 16 rows per (job, window): U = sum T_s + sum_j 2^(j + log_seg) Q_j. Row r takes
-// the terms r, r + 16, ...: term j < log_segs is Q_j doubled j + log_seg times, the last is sum T_s;
+// the terms r, r + 16, ...: a term is the sum of its reduction row's chunk partials; term
+// j < log_segs is Q_j doubled j + log_seg times, the last two are the halves of sum T_s;
 // the rows' sums then meet in a tree. The doublings of different terms run side by side (a Horner
 // pass over j would chain all of them with the additions), each row with the wave-parallel formulas.
 constexpr int kWinRows = 16;
 
 __global__ void __launch_bounds__(16 * kWinRows) k_window_sums(const MsmJob* __restrict__ jobs, int njobs,
-                                                               int64_t total_windows, const JacRec* __restrict__ rows,
+                                                               int64_t total_windows, const JacRec* __restrict__ parts,
                                                                JacRec* __restrict__ wsum) {
   __shared__ JacRec sh[kWinRows];
   const int64_t g = blockIdx.x;
   if (g >= total_windows) return;
   const MsmJob& job = jobs[find_job<&MsmJob::win_base>(jobs, njobs, g)];
   const int64_t win = g - job.win_base;
-  const int per = job.log_segs + 1;
-  const JacRec* p = rows + job.sums_base + win * per;
+  const int per = red_rows(job);
+  const JacRec* p = parts + job.part_base + win * per * job.chunks;
   const FqIlp f;
   const int row = (int)threadIdx.x >> 4;
   Jac acc = jac_identity(f);
   for (int t = row; t < per; t += kWinRows) {
-    Jac d = load_jac_rec(p + t);
+    Jac d = load_jac_rec(p + (int64_t)t * job.chunks);
+    for (int k = 1; k < job.chunks; ++k) d = jac_add_wave(f, d, load_jac_rec(p + (int64_t)t * job.chunks + k));
     const int dbl = t < job.log_segs ? t + job.log_seg : 0;
     for (int e = 0; e < dbl; ++e) d = jac_dbl_wave(f, d);
     acc = jac_add_wave(f, acc, d);
@@ -791,7 +790,7 @@ struct MsmPlan {
   int njobs = 0;
   std::vector<MsmJob> jobs;
   std::vector<int64_t> level_max;
-  int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, max_slots1 = 0, segments = 0, sums_rows = 0, parts = 0,
+  int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, max_slots1 = 0, segments = 0, parts = 0,
           windows_total = 0;
   int64_t sort_blocks = 0, acc_blocks = 0;
   size_t bytes = 0;
@@ -806,14 +805,14 @@ struct MsmPlan {
   uint2* sorted = nullptr;
   BucketRec* buckets = nullptr;
   RunSummary* runs[2] = {nullptr, nullptr};
-  JacRec *tot = nullptr, *run = nullptr, *parts_buf = nullptr, *rows_buf = nullptr, *wsum = nullptr;
+  JacRec *tot = nullptr, *run = nullptr, *parts_buf = nullptr, *wsum = nullptr;
 };
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Byte offsets of every buffer inside a plan's single device allocation.
 struct MsmLayout {
-  size_t jobs, counts, totals, starts, digits, m, sorted, scan, buckets, r0, r1, tot, run, parts, rows, wsum, total;
+  size_t jobs, counts, totals, starts, digits, m, sorted, scan, buckets, r0, r1, tot, run, parts, wsum, total;
 };
 
 // jobs: scalars, n, c (0 = default) and out filled in by the caller; the rest is planned here,
@@ -821,7 +820,7 @@ struct MsmLayout {
 static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk, MsmLayout* L) {
   plan->acc_variant = g_acc_variant;
   if (chunk < 0) return fail("chunk_len must be positive (or 0 for automatic)");
-  int64_t blocks = 0, keys = 0, entries = 0, segs = 0, sums = 0, parts = 0, wins = 0, sort_blocks = 0, cnts = 0;
+  int64_t blocks = 0, keys = 0, entries = 0, segs = 0, parts = 0, wins = 0, sort_blocks = 0, cnts = 0;
   for (auto& j : jobs) {
     if (j.n < 0 || j.n > 0x3fffffff) return fail("MSM size must be in [0, 2^30)");
     if (j.c <= 0) j.c = default_window(j.n);
@@ -832,11 +831,11 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
     j.rw = j.table ? 1 : j.windows;
     j.log_seg = std::min(2, j.c - 1);
     j.log_segs = (j.c - 1) - j.log_seg;
-    j.chunks = (int)(((1LL << j.log_segs) + kRedChunk - 1) / kRedChunk);
+    // chunks per reduction row: a row adds half the segments, or (the first T half) ceil(segs / 2)
+    j.chunks = (int)((((1LL << j.log_segs) + 1) / 2 + kRedChunk - 1) / kRedChunk);
     j.block_base = blocks;
     j.key_base = keys;
     j.seg_base = segs;
-    j.sums_base = sums;
     j.part_base = parts;
     j.win_base = wins;
     // sort geometry: windows x bucket ranges x scalar tiles, about a thousand blocks for large jobs
@@ -854,8 +853,7 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
     keys += (int64_t)j.rw << (j.c - 1);
     entries += (int64_t)j.windows * j.nv;
     segs += (int64_t)j.rw << j.log_segs;
-    sums += (int64_t)j.rw * (j.log_segs + 1);
-    parts += (int64_t)j.rw * (j.log_segs + 1) * j.chunks;
+    parts += (int64_t)j.rw * (j.log_segs + 2) * j.chunks;
     wins += j.rw;
   }
   if (keys >= 0x7fffffff || entries >= 0xffffffffLL) return fail("MSM batch too large for 32-bit bucket keys");
@@ -882,7 +880,6 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   plan->nkeys = keys;
   plan->max_entries = entries;
   plan->segments = segs;
-  plan->sums_rows = sums;
   plan->sort_blocks = sort_blocks;
   plan->parts = parts;
   plan->windows_total = wins;
@@ -917,7 +914,6 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   L->tot = take(sizeof(JacRec) * (size_t)segs);
   L->run = take(sizeof(JacRec) * (size_t)segs);
   L->parts = take(sizeof(JacRec) * (size_t)parts);
-  L->rows = take(sizeof(JacRec) * (size_t)sums);
   L->wsum = take(sizeof(JacRec) * (size_t)wins);
   L->total = off;
   return 0;
@@ -952,7 +948,6 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   plan->tot = reinterpret_cast<JacRec*>(base + L.tot);
   plan->run = reinterpret_cast<JacRec*>(base + L.run);
   plan->parts_buf = reinterpret_cast<JacRec*>(base + L.parts);
-  plan->rows_buf = reinterpret_cast<JacRec*>(base + L.rows);
   plan->wsum = reinterpret_cast<JacRec*>(base + L.wsum);
   // Stream-ordered upload from the plan's own job table, which lives as long as the plan.
   if (!plan->jobs.empty()) {
@@ -1022,11 +1017,8 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
     hipLaunchKernelGGL(k_reduce_bits, dim3((unsigned)plan->parts), dim3(256), 0, stream, plan->d_jobs, nj, plan->tot,
                        plan->run, plan->parts_buf);
     ZK_LAUNCH_CHECK("msm reduce bits");
-    hipLaunchKernelGGL(k_row_sums, dim3(grid_for(plan->sums_rows, 64)), dim3(64), 0, stream, plan->d_jobs, nj,
-                       plan->sums_rows, plan->parts_buf, plan->rows_buf);
-    ZK_LAUNCH_CHECK("msm row sums");
     hipLaunchKernelGGL(k_window_sums, dim3((unsigned)plan->windows_total), dim3(16 * kWinRows), 0, stream, plan->d_jobs, nj,
-                       plan->windows_total, plan->rows_buf, plan->wsum);
+                       plan->windows_total, plan->parts_buf, plan->wsum);
     ZK_LAUNCH_CHECK("msm window sums");
   }
   Prof prof("msm_final", stream);
