@@ -567,13 +567,87 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
   out[t] = me;
 }
 
-// Levels >= 1: the `count` summaries of the previous level, 256 per workgroup, merged by a tree.
+// The boundaries between consecutive summaries of one tree level, resolved before the next level
+// (one thread per summary t; its neighbours' keys decide). Each bucket cut by the boundaries is a
+// chain of fragments: the last run of a summary, the runs of summaries that hold nothing else, the
+// first run of a summary. A run bounded on both sides is written as it is; a chain of two fragments
+// is added up by the thread of its right fragment (one addition) and written; only chains of three
+// or more fragments (hot buckets, through whole summaries) stay for the trees, as reduced
+// summaries in `red` that hold just those fragments (one open run is marked single). Every other
+// reduced summary is empty, so the trees merge nothing else. Past the first level a summary covers
+// thousands of entries, so chains of three or more come from hot buckets only.
+__global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in, int64_t count,
+                                               BucketRec* __restrict__ buckets, RunSummary* __restrict__ red) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= count) return;
+  // keys and flags of the summaries t - 2 .. t + 2 (index 2 is t)
+  uint32_t fk[5], ke[5];
+  bool valid[5], single[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const int64_t i = t - 2 + k;
+    uint4 m = make_uint4(0, 0, 0, 0);
+    if (i >= 0 && i < count) m = *reinterpret_cast<const uint4*>(in + i);
+    fk[k] = m.x;
+    valid[k] = m.z & kValid;
+    single[k] = m.z & kSingle;
+    ke[k] = single[k] ? m.x : m.y;  // the key of the summary's last run
+  }
+  bool jl[5], jr[5];  // the summary's first (last) run continues in the previous (next) summary
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    jl[k] = k > 0 && valid[k - 1] && valid[k] && ke[k - 1] == fk[k];
+    jr[k] = k < 4 && valid[k] && valid[k + 1] && ke[k] == fk[k + 1];
+  }
+  if (!valid[2]) {
+    red[t].flags = 0;
+    return;
+  }
+  RunSummary r;
+  r.flags = 0;
+  const FqIlp f;
+  const RunSummary& me = in[t];
+  // the fragment that ends summary t - 1 (its single run or its last run)
+  auto left_end = [&]() { return load_jac_rec(single[1] ? &in[t - 1].fsum : &in[t - 1].lsum); };
+  if (single[2]) {
+    const bool lo = jl[2], hi = jr[2];
+    if (!lo && !hi) {
+      store_bucket(buckets + fk[2], load_jac_rec(&me.fsum));
+    } else if ((lo && hi) || (lo && single[1] && jl[1]) || (hi && single[3] && jr[3])) {
+      r.fk = r.lk = fk[2];
+      r.flags = kValid | kSingle;
+      r.fsum = me.fsum;
+    } else if (lo) {
+      store_bucket(buckets + fk[2], jac_add(f, left_end(), load_jac_rec(&me.fsum)));
+    }  // else: a chain of two that the next summary's thread adds up
+  } else {
+    bool first_in_tree = false, last_in_tree = false;
+    if (!jl[2]) store_bucket(buckets + fk[2], load_jac_rec(&me.fsum));
+    else if (single[1] && jl[1]) first_in_tree = true;
+    else store_bucket(buckets + fk[2], jac_add(f, left_end(), load_jac_rec(&me.fsum)));
+    if (!jr[2]) store_bucket(buckets + me.lk, load_jac_rec(&me.lsum));
+    else if (single[3] && jr[3]) last_in_tree = true;
+    if (first_in_tree && last_in_tree) {
+      r = me;
+    } else if (first_in_tree || last_in_tree) {
+      r.fk = r.lk = first_in_tree ? me.fk : me.lk;
+      r.flags = kValid | kSingle;
+      r.fsum = first_in_tree ? me.fsum : me.lsum;
+    }
+  }
+  if (r.flags) red[t] = r;
+  else red[t].flags = 0;
+}
+
+// The `count` summaries of the previous level, 256 per workgroup, merged by a tree (empty
+// summaries are not copied).
 __global__ void __launch_bounds__(kCombineThreads) k_combine(const RunSummary* __restrict__ in, int64_t count,
                                                              BucketRec* __restrict__ buckets, RunSummary* __restrict__ out,
                                                              int last_level) {
   __shared__ RunSummary sh[kCombineThreads];
   const int64_t t = (int64_t)blockIdx.x * kCombineThreads + threadIdx.x;
-  if (t < count) {
+  const uint32_t flags = t < count ? in[t].flags : 0u;
+  if (flags) {
     sh[threadIdx.x] = in[t];
   } else {
     sh[threadIdx.x].flags = 0;
@@ -1002,6 +1076,13 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
     Prof prof("msm_combine", stream);
     int cur = 0;
     for (int l = 1; l < levels; ++l) {
+      if (l == 2) {
+        // between the first and second tree levels: resolve the summaries' boundaries in one pass
+        hipLaunchKernelGGL(k_fixup, dim3(grid_for(plan->level_max[1], 256)), dim3(256), 0, stream, plan->runs[cur],
+                           plan->level_max[1], plan->buckets, plan->runs[cur ^ 1]);
+        ZK_LAUNCH_CHECK("msm boundaries");
+        cur ^= 1;
+      }
       hipLaunchKernelGGL(k_combine, dim3((unsigned)plan->level_max[(size_t)l]), dim3(kCombineThreads), 0, stream,
                          plan->runs[cur], plan->level_max[(size_t)l - 1], plan->buckets, plan->runs[cur ^ 1],
                          l + 1 == levels ? 1 : 0);
