@@ -574,8 +574,9 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
 // is added up by the thread of its right fragment (one addition) and written; only chains of three
 // or more fragments (hot buckets, through whole summaries) stay for the trees, as reduced
 // summaries in `red` that hold just those fragments (one open run is marked single). Every other
-// reduced summary is empty, so the trees merge nothing else. Past the first level a summary covers
-// thousands of entries, so chains of three or more come from hot buckets only.
+// reduced summary is empty, so the trees merge nothing else. When a thread's range is longer than
+// a typical bucket (the tuned windows make it so), and past the first level in any case, chains of
+// three or more come from hot buckets only.
 __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in, int64_t count,
                                                BucketRec* __restrict__ buckets, RunSummary* __restrict__ red) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -607,8 +608,11 @@ __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in
   r.flags = 0;
   const FqIlp f;
   const RunSummary& me = in[t];
-  // the fragment that ends summary t - 1 (its single run or its last run)
-  auto left_end = [&]() { return load_jac_rec(single[1] ? &in[t - 1].fsum : &in[t - 1].lsum); };
+  // the one addition a thread may owe: the first fragment of summary t (its single run or its
+  // first run) as the right end of a chain of two, added to the fragment that ends summary t - 1;
+  // every case below reaches the same call, so lanes that take different cases do not each pay
+  // for an addition of their own
+  bool add = false;
   if (single[2]) {
     const bool lo = jl[2], hi = jr[2];
     if (!lo && !hi) {
@@ -617,14 +621,14 @@ __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in
       r.fk = r.lk = fk[2];
       r.flags = kValid | kSingle;
       r.fsum = me.fsum;
-    } else if (lo) {
-      store_bucket(buckets + fk[2], jac_add(f, left_end(), load_jac_rec(&me.fsum)));
-    }  // else: a chain of two that the next summary's thread adds up
+    } else {
+      add = lo;  // else: a chain of two that the next summary's thread adds up
+    }
   } else {
     bool first_in_tree = false, last_in_tree = false;
     if (!jl[2]) store_bucket(buckets + fk[2], load_jac_rec(&me.fsum));
     else if (single[1] && jl[1]) first_in_tree = true;
-    else store_bucket(buckets + fk[2], jac_add(f, left_end(), load_jac_rec(&me.fsum)));
+    else add = true;
     if (!jr[2]) store_bucket(buckets + me.lk, load_jac_rec(&me.lsum));
     else if (single[3] && jr[3]) last_in_tree = true;
     if (first_in_tree && last_in_tree) {
@@ -634,6 +638,10 @@ __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in
       r.flags = kValid | kSingle;
       r.fsum = first_in_tree ? me.fsum : me.lsum;
     }
+  }
+  if (add) {
+    const JacRec* left = single[1] ? &in[t - 1].fsum : &in[t - 1].lsum;
+    store_bucket(buckets + fk[2], jac_add(f, load_jac_rec(left), load_jac_rec(&me.fsum)));
   }
   if (r.flags) red[t] = r;
   else red[t].flags = 0;
@@ -964,9 +972,11 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   while (plan->level_max.back() > 1) plan->level_max.push_back((plan->level_max.back() + kCombineThreads - 1) / kCombineThreads);
   int64_t slots[2] = {1, 1};
   for (size_t l = 0; l < plan->level_max.size(); ++l) slots[l & 1] = std::max(slots[l & 1], plan->level_max[l]);
-  plan->max_slots = slots[0];
-  plan->max_slots1 = slots[1];
-  const int64_t max_slots = slots[0], max_slots1 = slots[1];
+  // the boundary pass before the first tree level reads level 0 from one buffer and writes as many
+  // reduced summaries into the other, so both hold the level-0 count
+  const int64_t max_slots = slots[0], max_slots1 = std::max(slots[1], slots[0]);
+  plan->max_slots = max_slots;
+  plan->max_slots1 = max_slots1;
   const size_t e1 = (size_t)std::max<int64_t>(entries, 1);
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -1066,6 +1076,7 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
     ZK_LAUNCH_CHECK("msm sort place");
   }
   const int levels = (int)plan->level_max.size();
+  if (plan->level_max[0] > plan->max_slots) return fail("internal: MSM run-summary buffer smaller than a level");
   {
     Prof prof("msm_accumulate", stream);
     hipLaunchKernelGGL(acc_kernel(plan->acc_variant), dim3((unsigned)plan->acc_blocks), dim3(kAccThreads), 0, stream, plan->sorted,
@@ -1074,12 +1085,16 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
   }
   {
     Prof prof("msm_combine", stream);
+    // every pass writes the other summary buffer: check the count against its capacity first
+    const int64_t cap[2] = {plan->max_slots, plan->max_slots1};
     int cur = 0;
     for (int l = 1; l < levels; ++l) {
-      if (l == 2) {
-        // between the first and second tree levels: resolve the summaries' boundaries in one pass
-        hipLaunchKernelGGL(k_fixup, dim3(grid_for(plan->level_max[1], 256)), dim3(256), 0, stream, plan->runs[cur],
-                           plan->level_max[1], plan->buckets, plan->runs[cur ^ 1]);
+      if ((l <= 2 && plan->level_max[(size_t)l - 1] > cap[cur ^ 1]) || plan->level_max[(size_t)l] > cap[(l <= 2 ? cur : cur ^ 1)])
+        return fail("internal: MSM run-summary buffer smaller than a level");
+      if (l <= 2) {
+        // before the first and second tree levels: resolve the summaries' boundaries in one pass
+        hipLaunchKernelGGL(k_fixup, dim3(grid_for(plan->level_max[l - 1], 256)), dim3(256), 0, stream,
+                           plan->runs[cur], plan->level_max[l - 1], plan->buckets, plan->runs[cur ^ 1]);
         ZK_LAUNCH_CHECK("msm boundaries");
         cur ^= 1;
       }
