@@ -206,3 +206,21 @@ def test_accumulation_variants(variant):
             assert (got.is_identity() and e == 0) or got.affine() == (G * e).affine(), (variant, chunk)
     finally:
         gpu_backend.call("zk_msm_set_accumulate", old.value)
+
+
+@pytest.mark.parametrize("chunk", [0, 2, 5])
+def test_one_bucket_spans_every_range(chunk):
+    """Scalars that send (almost) every entry to one bucket: the bucket's run crosses every range
+    boundary of the accumulation, so its fragments go through both boundary passes and every
+    level of the run-summary trees (with short ranges, several levels of them)."""
+    n = 1 << 15
+    keys = workloads.srs_keys(n, 0x5EED0E)
+    pts = workloads.srs_points(ints_to_limbs(keys))
+    rng = random.Random(43)
+    cases = [[1] * n, [1 if rng.random() < 0.9 else rng.randrange(R) for _ in range(n)]]
+    for scalars in cases:
+        e = workloads.closed_form_scalar(keys, scalars)
+        out = gpu_backend.empty((3, 4))
+        msm_gpu.msm_g1_jacobian(pts, gpu_backend.asarray(ints_to_limbs(scalars)), out, chunk_len=chunk)
+        got = msm_gpu._result(out)
+        assert (got.is_identity() and e == 0) or got.affine() == (G * e).affine(), chunk
