@@ -231,8 +231,9 @@ struct MsmJob {
   int64_t part_base;        // first (window, bit, chunk) partial of the bucket reduction
   int64_t win_base;         // first per-window sum
   int64_t dig_base;         // first signed digit (window-major: window w at dig_base + w * n)
-  int64_t sort_base;        // first sort block (window, bucket range, scalar tile)
-  int64_t cnt_base;         // first (bucket, tile) count
+  int64_t sort_base;        // first sort block (window, scalar tile)
+  int64_t cnt_base;         // first (partition, tile) count
+  int64_t pkey_base;        // first partition key (a partition: 2^pbits consecutive buckets of a window)
   int64_t tile;             // scalars per sort tile
   uint64_t* out;            // 12 limbs, the Jacobian result
   int64_t nv;               // digit streams: n, or 2n with the GLV split (k1 for P_i, k2 for phi(P_i))
@@ -240,6 +241,7 @@ struct MsmJob {
   int64_t table_base;       // with a window table: record index of 2^(c w) P_i at table_base + w * stride + i
   int64_t table_stride;
   int c, windows, log_seg, log_segs, chunks, tiles, ranges, glv;
+  int pbits;                // bucket bits inside a partition: min(c - 1, kPartBits)
   int table;                // 1: every window's digits share one set of buckets (the SRS's window table)
   int rw;                   // windows the bucket reduction and the final Horner chain see (1 with a table)
 };
@@ -288,12 +290,20 @@ __global__ void __launch_bounds__(256) k_signed_digits(const MsmJob* __restrict_
   }
 }
 
-constexpr int kSortRange = 16384;  // buckets per sort block (its LDS holds one counter per bucket)
+// The sort runs in two passes so that its writes stay local. Pass A is a counting sort of the
+// digits by partition (a window's buckets in groups of 2^kPartBits): a workgroup's tile of digits
+// goes to a few dozen partitions, so its records leave in runs of hundreds. Pass B sorts each
+// partition by bucket in chunks of kChunkB records, writing inside the partition's own region.
+// (One counting sort by bucket would scatter single 8-byte records over the whole array: each
+// costs a memory transaction of its own.)
+constexpr int kPartBits = 9;              // buckets per partition: 512
+constexpr int kPartMax = 1 << 11;         // partitions per window at most (c <= 20)
+constexpr int kChunkB = 8192;             // records per pass-B workgroup
 
-// Sort block geometry: (window, bucket range, scalar tile) of one job.
+// Sort block geometry: (window, scalar tile) of one job.
 struct SortBlock {
   const MsmJob* job;
-  int win, range, tile;
+  int win, tile;
 };
 
 __device__ __forceinline__ SortBlock sort_block(const MsmJob* __restrict__ jobs, int njobs) {
@@ -302,87 +312,201 @@ __device__ __forceinline__ SortBlock sort_block(const MsmJob* __restrict__ jobs,
   SortBlock b;
   b.job = job;
   b.tile = (int)(local % job->tiles);
-  const int64_t rest = local / job->tiles;
-  b.range = (int)(rest % job->ranges);
-  b.win = (int)(rest / job->ranges);
+  b.win = (int)(local / job->tiles);
   return b;
 }
 
-// Counts, per (bucket, tile), the digits of one tile that fall into this block's bucket range.
-// Counts are laid out tile-major (window w, tile t: a row of 2^(c-1) buckets), so every
-// workgroup writes its range as one contiguous run.
+__device__ __forceinline__ int parts_per_window(const MsmJob& job) { return 1 << (job.c - 1 - job.pbits); }
+
+// Pass A, counts: per (partition, tile), the nonzero digits of one tile of one window. Counts are
+// laid out tile-major (window w, tile t: a row of the window's partitions).
 __global__ void __launch_bounds__(256) k_sort_count(const MsmJob* __restrict__ jobs, int njobs,
                                                     const int32_t* __restrict__ digits, uint32_t* __restrict__ counts) {
-  extern __shared__ uint32_t hist[];
+  __shared__ uint32_t hist[kPartMax];
   const SortBlock sb = sort_block(jobs, njobs);
   const MsmJob& job = *sb.job;
-  const int64_t buckets = 1LL << (job.c - 1);
-  const int64_t r0 = (int64_t)sb.range * kSortRange;
-  const int range = (int)min((int64_t)kSortRange, buckets - r0);
-  for (int k = threadIdx.x; k < range; k += 256) hist[k] = 0;
+  const int np = parts_per_window(job);
+  for (int k = threadIdx.x; k < np; k += 256) hist[k] = 0;
   __syncthreads();
   const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.nv, lo + job.tile);
   const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.nv;
   for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
     const int32_t v = d[i];
-    const int64_t b = (int64_t)(v < 0 ? -v : v) - 1 - r0;  // bucket index within the range
-    if (v != 0 && b >= 0 && b < range) atomicAdd(&hist[b], 1u);
+    if (v != 0) atomicAdd(&hist[((v < 0 ? -v : v) - 1) >> job.pbits], 1u);
   }
   __syncthreads();
-  uint32_t* out = counts + job.cnt_base + ((int64_t)sb.win * job.tiles + sb.tile) * buckets + r0;
-  for (int k = threadIdx.x; k < range; k += 256) out[k] = hist[k];
+  uint32_t* out = counts + job.cnt_base + ((int64_t)sb.win * job.tiles + sb.tile) * np;
+  for (int k = threadIdx.x; k < np; k += 256) out[k] = hist[k];
 }
 
-// One thread per bucket key: turns the bucket's per-tile counts into per-tile offsets inside the
-// bucket (an exclusive scan down the tiles, coalesced across neighbouring buckets) and writes
-// the bucket's total, whose scan over all keys gives every bucket its first sorted position.
-__global__ void __launch_bounds__(256) k_sort_columns(const MsmJob* __restrict__ jobs, int njobs, int64_t nkeys,
+// One thread per partition key: turns the partition's per-tile counts into per-tile offsets inside
+// the partition (an exclusive scan down the tiles) and writes its total, whose scan over all
+// partitions gives every partition its first sorted position.
+__global__ void __launch_bounds__(256) k_sort_columns(const MsmJob* __restrict__ jobs, int njobs, int64_t nparts,
                                                       uint32_t* __restrict__ counts, uint32_t* __restrict__ totals) {
   const int64_t key = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (key >= nkeys) return;
-  const MsmJob& job = jobs[find_job<&MsmJob::key_base>(jobs, njobs, key)];
-  const int64_t local = key - job.key_base;
-  const int64_t buckets = 1LL << (job.c - 1);
-  const int64_t win = local >> (job.c - 1), b = local & (buckets - 1);
-  // a key's rows: the tiles of its window, or of every window when they share the buckets
-  uint32_t* col = counts + job.cnt_base + (job.table ? 0 : win * job.tiles * buckets) + b;
+  if (key >= nparts) return;
+  const MsmJob& job = jobs[find_job<&MsmJob::pkey_base>(jobs, njobs, key)];
+  const int np = parts_per_window(job);
+  const int64_t local = key - job.pkey_base;
+  const int64_t win = local / np, q = local % np;
+  // a partition's rows: the tiles of its window, or of every window when they share the buckets
+  uint32_t* col = counts + job.cnt_base + (job.table ? 0 : win * job.tiles * np) + q;
   const int rows = job.table ? job.windows * job.tiles : job.tiles;
   uint32_t run = 0;
   for (int t = 0; t < rows; ++t) {
-    const uint32_t v = col[(int64_t)t * buckets];
-    col[(int64_t)t * buckets] = run;
+    const uint32_t v = col[(int64_t)t * np];
+    col[(int64_t)t * np] = run;
     run += v;
   }
   totals[key] = run;
 }
 
-// Places each digit's (key, point ref) record at its bucket's next free position: the block's LDS
-// holds the running positions of its (bucket, tile) slots, so no global atomics are needed.
+// Pass A, placement: each digit's (bucket key, point ref) record goes to its partition's next
+// free position (LDS cursors per partition, no global atomics).
 __global__ void __launch_bounds__(256) k_sort_place(const MsmJob* __restrict__ jobs, int njobs,
                                                     const int32_t* __restrict__ digits, const uint32_t* __restrict__ counts,
-                                                    const uint32_t* __restrict__ starts, uint2* __restrict__ sorted) {
-  extern __shared__ uint32_t cur[];
+                                                    const uint32_t* __restrict__ starts, uint2* __restrict__ staged) {
+  __shared__ uint32_t cur[kPartMax];
   const SortBlock sb = sort_block(jobs, njobs);
   const MsmJob& job = *sb.job;
+  const int np = parts_per_window(job);
   const int64_t buckets = 1LL << (job.c - 1);
-  const int64_t r0 = (int64_t)sb.range * kSortRange;
-  const int range = (int)min((int64_t)kSortRange, buckets - r0);
-  const uint32_t* off = counts + job.cnt_base + ((int64_t)sb.win * job.tiles + sb.tile) * buckets + r0;
-  const int64_t key0 = job.key_base + (job.table ? 0 : (int64_t)sb.win * buckets) + r0;
-  for (int k = threadIdx.x; k < range; k += 256) cur[k] = starts[key0 + k] + off[k];
+  const uint32_t* off = counts + job.cnt_base + ((int64_t)sb.win * job.tiles + sb.tile) * np;
+  const int64_t pkey0 = job.pkey_base + (job.table ? 0 : (int64_t)sb.win * np);
+  const int64_t key0 = job.key_base + (job.table ? 0 : (int64_t)sb.win * buckets);
+  for (int k = threadIdx.x; k < np; k += 256) cur[k] = starts[pkey0 + k] + off[k];
   __syncthreads();
   const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.nv, lo + job.tile);
   const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.nv;
   for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
     const int32_t v = d[i];
-    const int64_t b = (int64_t)(v < 0 ? -v : v) - 1 - r0;
-    if (v != 0 && b >= 0 && b < range) {
-      const uint32_t pos = atomicAdd(&cur[b], 1u);
+    if (v != 0) {
+      const int64_t b = (int64_t)(v < 0 ? -v : v) - 1;
+      const uint32_t pos = atomicAdd(&cur[b >> job.pbits], 1u);
       // the point record: 2^(c w) P_i from the window table, P_i, or phi(P_i) for the k2 stream
       const int64_t rec = job.table ? job.table_base + (int64_t)sb.win * job.table_stride + i
                                     : (i < job.n ? i : job.phi_base + (i - job.n));
-      sorted[pos] = make_uint2((uint32_t)(key0 + b), (uint32_t)rec | (v < 0 ? 0x80000000u : 0u));
+      staged[pos] = make_uint2((uint32_t)(key0 + b), (uint32_t)rec | (v < 0 ? 0x80000000u : 0u));
     }
+  }
+}
+
+// Pass B geometry: chunks of kChunkB records per partition, numbered partition after partition.
+__global__ void __launch_bounds__(256) k_part_chunks(const uint32_t* __restrict__ totals, int64_t nparts,
+                                                     uint32_t* __restrict__ chunks) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nparts) chunks[p] = (totals[p] + kChunkB - 1) / kChunkB;
+}
+
+// The (partition, chunk) of pass-B workgroup g, or false past the last chunk.
+struct PartChunk {
+  int64_t p, lo, hi, bkey0;
+  int pbits;
+};
+
+__device__ __forceinline__ bool part_chunk(const MsmJob* __restrict__ jobs, int njobs, const uint32_t* __restrict__ chunk_base,
+                                           int64_t nparts, const uint32_t* __restrict__ starts,
+                                           const uint32_t* __restrict__ totals, int64_t g, PartChunk* pc) {
+  if (g >= (int64_t)chunk_base[nparts]) return false;
+  int64_t lo = 0, hi = nparts;  // chunk_base[lo] <= g < chunk_base[hi]
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)chunk_base[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const MsmJob& job = jobs[find_job<&MsmJob::pkey_base>(jobs, njobs, lo)];
+  pc->p = lo;
+  pc->pbits = job.pbits;
+  pc->bkey0 = job.key_base + ((lo - job.pkey_base) << job.pbits);
+  const int64_t first = (int64_t)starts[lo] + (g - (int64_t)chunk_base[lo]) * kChunkB;
+  pc->lo = first;
+  pc->hi = min(first + (int64_t)kChunkB, (int64_t)starts[lo] + (int64_t)totals[lo]);
+  return true;
+}
+
+// Pass B, counts: per (chunk, bucket of its partition).
+__global__ void __launch_bounds__(256) k_part_count(const MsmJob* __restrict__ jobs, int njobs,
+                                                    const uint32_t* __restrict__ chunk_base, int64_t nparts,
+                                                    const uint32_t* __restrict__ starts, const uint32_t* __restrict__ totals,
+                                                    const uint2* __restrict__ staged, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t hist[1 << kPartBits];
+  PartChunk pc;
+  if (!part_chunk(jobs, njobs, chunk_base, nparts, starts, totals, blockIdx.x, &pc)) return;
+  const int nb = 1 << pc.pbits;
+  for (int k = threadIdx.x; k < nb; k += 256) hist[k] = 0;
+  __syncthreads();
+  for (int64_t x = pc.lo + threadIdx.x; x < pc.hi; x += 256) atomicAdd(&hist[staged[x].x - pc.bkey0], 1u);
+  __syncthreads();
+  uint32_t* out = counts + (int64_t)blockIdx.x * (1 << kPartBits);
+  for (int k = threadIdx.x; k < nb; k += 256) out[k] = hist[k];
+}
+
+// Pass B, offsets: one workgroup per partition. Per bucket an exclusive scan down the partition's
+// chunks, then a scan over the buckets; every (chunk, bucket) count becomes the absolute position
+// of the chunk's first record of that bucket.
+__global__ void __launch_bounds__(256) k_part_columns(const MsmJob* __restrict__ jobs, int njobs,
+                                                      const uint32_t* __restrict__ chunk_base, int64_t nparts,
+                                                      const uint32_t* __restrict__ starts, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t part[256];
+  const int64_t p = blockIdx.x;
+  if (p >= nparts) return;
+  const MsmJob& job = jobs[find_job<&MsmJob::pkey_base>(jobs, njobs, p)];
+  const int nb = 1 << job.pbits;
+  const int64_t g0 = chunk_base[p], g1 = chunk_base[p + 1];
+  constexpr int kPer = (1 << kPartBits) / 256;  // buckets per thread
+  uint32_t run[kPer];
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    const int j = threadIdx.x * kPer + e;
+    run[e] = 0;
+    if (j < nb) {
+      for (int64_t g = g0; g < g1; ++g) {
+        uint32_t* c = counts + g * (1 << kPartBits) + j;
+        const uint32_t v = *c;
+        *c = run[e];
+        run[e] += v;
+      }
+    }
+  }
+  // exclusive scan of the bucket totals: per thread, then over the 256 threads
+  uint32_t sum = 0;
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) sum += run[e];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const uint32_t v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t base = starts[p] + part[threadIdx.x] - sum;  // exclusive
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    const int j = threadIdx.x * kPer + e;
+    if (j < nb)
+      for (int64_t g = g0; g < g1; ++g) counts[g * (1 << kPartBits) + j] += base;
+    base += run[e];
+  }
+}
+
+// Pass B, placement: each staged record of the chunk to its bucket's next position.
+__global__ void __launch_bounds__(256) k_part_place(const MsmJob* __restrict__ jobs, int njobs,
+                                                    const uint32_t* __restrict__ chunk_base, int64_t nparts,
+                                                    const uint32_t* __restrict__ starts, const uint32_t* __restrict__ totals,
+                                                    const uint2* __restrict__ staged, const uint32_t* __restrict__ counts,
+                                                    uint2* __restrict__ sorted) {
+  __shared__ uint32_t cur[1 << kPartBits];
+  PartChunk pc;
+  if (!part_chunk(jobs, njobs, chunk_base, nparts, starts, totals, blockIdx.x, &pc)) return;
+  const int nb = 1 << pc.pbits;
+  const uint32_t* in = counts + (int64_t)blockIdx.x * (1 << kPartBits);
+  for (int k = threadIdx.x; k < nb; k += 256) cur[k] = in[k];
+  __syncthreads();
+  for (int64_t x = pc.lo + threadIdx.x; x < pc.hi; x += 256) {
+    const uint2 r = staged[x];
+    sorted[atomicAdd(&cur[r.x - pc.bkey0], 1u)] = r;
   }
 }
 
@@ -875,6 +999,7 @@ struct MsmPlan {
   int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, max_slots1 = 0, segments = 0, parts = 0,
           windows_total = 0;
   int64_t sort_blocks = 0, acc_blocks = 0;
+  int64_t nparts = 0, max_chunks = 0;  // sort partitions; pass-B chunks at most
   size_t bytes = 0;
   void* srs = nullptr;  // the SRS handle the plan reads, if any
   int acc_variant = 0;  // the accumulation kernel (g_acc_variant when planned)
@@ -884,7 +1009,8 @@ struct MsmPlan {
   void* mem = nullptr;
   MsmJob* d_jobs = nullptr;
   uint32_t *counts = nullptr, *totals = nullptr, *starts = nullptr, *m_dev = nullptr, *scan_ws = nullptr;
-  uint2* sorted = nullptr;
+  uint32_t *chunks = nullptr, *chunk_base = nullptr, *counts_b = nullptr;
+  uint2 *staged = nullptr, *sorted = nullptr;
   BucketRec* buckets = nullptr;
   RunSummary* runs[2] = {nullptr, nullptr};
   JacRec *tot = nullptr, *run = nullptr, *parts_buf = nullptr, *wsum = nullptr;
@@ -894,7 +1020,8 @@ static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Byte offsets of every buffer inside a plan's single device allocation.
 struct MsmLayout {
-  size_t jobs, counts, totals, starts, digits, m, sorted, scan, buckets, r0, r1, tot, run, parts, wsum, total;
+  size_t jobs, counts, totals, starts, digits, m, staged, sorted, scan, chunks, chunk_base, counts_b, buckets, r0, r1, tot, run,
+      parts, wsum, total;
 };
 
 // jobs: scalars, n, c (0 = default) and out filled in by the caller; the rest is planned here,
@@ -902,7 +1029,7 @@ struct MsmLayout {
 static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk, MsmLayout* L) {
   plan->acc_variant = g_acc_variant;
   if (chunk < 0) return fail("chunk_len must be positive (or 0 for automatic)");
-  int64_t blocks = 0, keys = 0, entries = 0, segs = 0, parts = 0, wins = 0, sort_blocks = 0, cnts = 0;
+  int64_t blocks = 0, keys = 0, entries = 0, segs = 0, parts = 0, wins = 0, sort_blocks = 0, cnts = 0, pkeys = 0;
   for (auto& j : jobs) {
     if (j.n < 0 || j.n > 0x3fffffff) return fail("MSM size must be in [0, 2^30)");
     if (j.c <= 0) j.c = default_window(j.n);
@@ -920,17 +1047,21 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
     j.seg_base = segs;
     j.part_base = parts;
     j.win_base = wins;
-    // sort geometry: windows x bucket ranges x scalar tiles, about a thousand blocks for large jobs
-    const int64_t nb = 1LL << (j.c - 1);
-    j.ranges = (int)((nb + kSortRange - 1) / kSortRange);
-    const int64_t want = (1024 + (int64_t)j.windows * j.ranges - 1) / ((int64_t)j.windows * j.ranges);
+    // sort geometry: windows x scalar tiles, about a thousand blocks for large jobs (pass A), and
+    // partitions of 2^pbits buckets (pass B)
+    j.pbits = std::min(j.c - 1, kPartBits);
+    const int64_t np = 1LL << (j.c - 1 - j.pbits);
+    j.ranges = 1;
+    const int64_t want = (1024 + (int64_t)j.windows - 1) / j.windows;
     j.tiles = (int)std::max<int64_t>(1, std::min<int64_t>(want, (j.nv + 2047) / 2048));
     j.tile = std::max<int64_t>(1, (j.nv + j.tiles - 1) / j.tiles);
     j.dig_base = entries;
     j.sort_base = sort_blocks;
     j.cnt_base = cnts;
-    sort_blocks += (int64_t)j.windows * j.ranges * j.tiles;
-    cnts += (int64_t)j.windows * nb * j.tiles;
+    j.pkey_base = pkeys;
+    sort_blocks += (int64_t)j.windows * j.tiles;
+    cnts += (int64_t)j.windows * np * j.tiles;
+    pkeys += (int64_t)j.rw * np;
     blocks += std::max<int64_t>(1, (j.n + 255) / 256);
     keys += (int64_t)j.rw << (j.c - 1);
     entries += (int64_t)j.windows * j.nv;
@@ -960,6 +1091,8 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   plan->njobs = (int)jobs.size();
   plan->blocks = blocks;
   plan->nkeys = keys;
+  plan->nparts = pkeys;
+  plan->max_chunks = (entries + kChunkB - 1) / kChunkB + pkeys;  // every partition adds at most one partial chunk
   plan->max_entries = entries;
   plan->segments = segs;
   plan->sort_blocks = sort_blocks;
@@ -986,12 +1119,16 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   };
   L->jobs = take(sizeof(MsmJob) * jobs.size());
   L->counts = take(4 * (size_t)std::max<int64_t>(cnts, 1));
-  L->totals = take(4 * (size_t)(keys + 1));
-  L->starts = take(4 * (size_t)(keys + 1));
+  L->totals = take(4 * (size_t)(pkeys + 1));
+  L->starts = take(4 * (size_t)(pkeys + 1));
   L->digits = take(4 * e1);
   L->m = take(16);
+  L->staged = take(8 * e1);
   L->sorted = take(8 * e1);
-  L->scan = take(4 * scan_workspace_words(keys));
+  L->scan = take(4 * scan_workspace_words(pkeys));
+  L->chunks = take(4 * (size_t)(pkeys + 1));
+  L->chunk_base = take(4 * (size_t)(pkeys + 1));
+  L->counts_b = take(4 * ((size_t)plan->max_chunks << kPartBits));
   L->buckets = take(sizeof(BucketRec) * (size_t)keys);
   L->r0 = take(sizeof(RunSummary) * (size_t)max_slots);
   L->r1 = take(sizeof(RunSummary) * (size_t)max_slots1);
@@ -1024,7 +1161,11 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   plan->starts = reinterpret_cast<uint32_t*>(base + L.starts);
   plan->digits = reinterpret_cast<int32_t*>(base + L.digits);
   plan->m_dev = reinterpret_cast<uint32_t*>(base + L.m);
+  plan->staged = reinterpret_cast<uint2*>(base + L.staged);
   plan->sorted = reinterpret_cast<uint2*>(base + L.sorted);
+  plan->chunks = reinterpret_cast<uint32_t*>(base + L.chunks);
+  plan->chunk_base = reinterpret_cast<uint32_t*>(base + L.chunk_base);
+  plan->counts_b = reinterpret_cast<uint32_t*>(base + L.counts_b);
   plan->scan_ws = reinterpret_cast<uint32_t*>(base + L.scan);
   plan->buckets = reinterpret_cast<BucketRec*>(base + L.buckets);
   plan->runs[0] = reinterpret_cast<RunSummary*>(base + L.r0);
@@ -1064,16 +1205,32 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
     Prof prof("msm_sort", stream);
     hipLaunchKernelGGL(k_signed_digits, dim3((unsigned)plan->blocks), dim3(256), 0, stream, plan->d_jobs, nj, plan->digits);
     ZK_LAUNCH_CHECK("msm digits");
-    hipLaunchKernelGGL(k_sort_count, dim3((unsigned)plan->sort_blocks), dim3(256), kSortRange * 4, stream, plan->d_jobs, nj,
+    hipLaunchKernelGGL(k_sort_count, dim3((unsigned)plan->sort_blocks), dim3(256), 0, stream, plan->d_jobs, nj,
                        plan->digits, plan->counts);
     ZK_LAUNCH_CHECK("msm sort count");
-    hipLaunchKernelGGL(k_sort_columns, dim3(grid_for(plan->nkeys, 256)), dim3(256), 0, stream, plan->d_jobs, nj,
-                       plan->nkeys, plan->counts, plan->totals);
+    hipLaunchKernelGGL(k_sort_columns, dim3(grid_for(plan->nparts, 256)), dim3(256), 0, stream, plan->d_jobs, nj,
+                       plan->nparts, plan->counts, plan->totals);
     ZK_LAUNCH_CHECK("msm sort columns");
-    if (int rc = scan_u32_ws(plan->totals, plan->starts, plan->nkeys, plan->m_dev, plan->scan_ws, stream)) return rc;
-    hipLaunchKernelGGL(k_sort_place, dim3((unsigned)plan->sort_blocks), dim3(256), kSortRange * 4, stream, plan->d_jobs, nj,
-                       plan->digits, plan->counts, plan->starts, plan->sorted);
+    if (int rc = scan_u32_ws(plan->totals, plan->starts, plan->nparts, plan->m_dev, plan->scan_ws, stream)) return rc;
+    hipLaunchKernelGGL(k_sort_place, dim3((unsigned)plan->sort_blocks), dim3(256), 0, stream, plan->d_jobs, nj,
+                       plan->digits, plan->counts, plan->starts, plan->staged);
     ZK_LAUNCH_CHECK("msm sort place");
+    // pass B: every partition sorted by bucket, chunk by chunk
+    hipLaunchKernelGGL(k_part_chunks, dim3(grid_for(plan->nparts, 256)), dim3(256), 0, stream, plan->totals, plan->nparts,
+                       plan->chunks);
+    ZK_LAUNCH_CHECK("msm sort chunks");
+    if (int rc = scan_u32_ws(plan->chunks, plan->chunk_base, plan->nparts, plan->chunk_base + plan->nparts, plan->scan_ws,
+                             stream))
+      return rc;
+    hipLaunchKernelGGL(k_part_count, dim3((unsigned)plan->max_chunks), dim3(256), 0, stream, plan->d_jobs, nj, plan->chunk_base,
+                       plan->nparts, plan->starts, plan->totals, plan->staged, plan->counts_b);
+    ZK_LAUNCH_CHECK("msm sort partition count");
+    hipLaunchKernelGGL(k_part_columns, dim3((unsigned)plan->nparts), dim3(256), 0, stream, plan->d_jobs, nj, plan->chunk_base,
+                       plan->nparts, plan->starts, plan->counts_b);
+    ZK_LAUNCH_CHECK("msm sort partition columns");
+    hipLaunchKernelGGL(k_part_place, dim3((unsigned)plan->max_chunks), dim3(256), 0, stream, plan->d_jobs, nj, plan->chunk_base,
+                       plan->nparts, plan->starts, plan->totals, plan->staged, plan->counts_b, plan->sorted);
+    ZK_LAUNCH_CHECK("msm sort partition place");
   }
   const int levels = (int)plan->level_max.size();
   if (plan->level_max[0] > plan->max_slots) return fail("internal: MSM run-summary buffer smaller than a level");
