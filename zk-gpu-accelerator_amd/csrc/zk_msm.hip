@@ -339,12 +339,13 @@ __global__ void __launch_bounds__(256) k_sort_count(const MsmJob* __restrict__ j
   for (int k = threadIdx.x; k < np; k += 256) out[k] = hist[k];
 }
 
-// One thread per partition key: turns the partition's per-tile counts into per-tile offsets inside
-// the partition (an exclusive scan down the tiles) and writes its total, whose scan over all
-// partitions gives every partition its first sorted position.
+// One wavefront per partition key: turns the partition's per-tile counts into per-tile offsets
+// inside the partition (an exclusive scan down the tiles, 64 tiles per step across the lanes) and
+// writes its total, whose scan over all partitions gives every partition its first sorted position.
 __global__ void __launch_bounds__(256) k_sort_columns(const MsmJob* __restrict__ jobs, int njobs, int64_t nparts,
                                                       uint32_t* __restrict__ counts, uint32_t* __restrict__ totals) {
-  const int64_t key = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t key = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
   if (key >= nparts) return;
   const MsmJob& job = jobs[find_job<&MsmJob::pkey_base>(jobs, njobs, key)];
   const int np = parts_per_window(job);
@@ -353,13 +354,20 @@ __global__ void __launch_bounds__(256) k_sort_columns(const MsmJob* __restrict__
   // a partition's rows: the tiles of its window, or of every window when they share the buckets
   uint32_t* col = counts + job.cnt_base + (job.table ? 0 : win * job.tiles * np) + q;
   const int rows = job.table ? job.windows * job.tiles : job.tiles;
-  uint32_t run = 0;
-  for (int t = 0; t < rows; ++t) {
-    const uint32_t v = col[(int64_t)t * np];
-    col[(int64_t)t * np] = run;
-    run += v;
+  uint32_t run = 0;  // the same in every lane
+  for (int t0 = 0; t0 < rows; t0 += 64) {
+    const int t = t0 + lane;
+    const uint32_t v = t < rows ? col[(int64_t)t * np] : 0u;
+    uint32_t incl = v;  // inclusive scan across the lanes
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += o;
+    }
+    if (t < rows) col[(int64_t)t * np] = run + incl - v;
+    run += __shfl(incl, 63, 64);
   }
-  totals[key] = run;
+  if (lane == 0) totals[key] = run;
 }
 
 // Pass A, placement: each digit's (bucket key, point ref) record goes to its partition's next
@@ -1208,7 +1216,7 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
     hipLaunchKernelGGL(k_sort_count, dim3((unsigned)plan->sort_blocks), dim3(256), 0, stream, plan->d_jobs, nj,
                        plan->digits, plan->counts);
     ZK_LAUNCH_CHECK("msm sort count");
-    hipLaunchKernelGGL(k_sort_columns, dim3(grid_for(plan->nparts, 256)), dim3(256), 0, stream, plan->d_jobs, nj,
+    hipLaunchKernelGGL(k_sort_columns, dim3(grid_for(plan->nparts * 64, 256)), dim3(256), 0, stream, plan->d_jobs, nj,
                        plan->nparts, plan->counts, plan->totals);
     ZK_LAUNCH_CHECK("msm sort columns");
     if (int rc = scan_u32_ws(plan->totals, plan->starts, plan->nparts, plan->m_dev, plan->scan_ws, stream)) return rc;
