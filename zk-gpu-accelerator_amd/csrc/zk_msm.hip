@@ -499,23 +499,68 @@ __global__ void __launch_bounds__(256) k_part_columns(const MsmJob* __restrict__
   }
 }
 
-// Pass B, placement: each staged record of the chunk to its bucket's next position.
+// Pass B, placement: the chunk's records are sorted by bucket in LDS first (a histogram, its scan,
+// and LDS cursors), then leave in that order, so that a bucket's records of the chunk are
+// consecutive in memory and written as runs rather than one by one.
 __global__ void __launch_bounds__(256) k_part_place(const MsmJob* __restrict__ jobs, int njobs,
                                                     const uint32_t* __restrict__ chunk_base, int64_t nparts,
                                                     const uint32_t* __restrict__ starts, const uint32_t* __restrict__ totals,
                                                     const uint2* __restrict__ staged, const uint32_t* __restrict__ counts,
                                                     uint2* __restrict__ sorted) {
-  __shared__ uint32_t cur[1 << kPartBits];
+  constexpr int kB = 1 << kPartBits;
+  __shared__ uint2 buf[kChunkB];
+  __shared__ uint32_t first[kB];  // the bucket's first slot in buf
+  __shared__ uint32_t cur[kB];
+  __shared__ uint32_t part[256];
   PartChunk pc;
   if (!part_chunk(jobs, njobs, chunk_base, nparts, starts, totals, blockIdx.x, &pc)) return;
   const int nb = 1 << pc.pbits;
-  const uint32_t* in = counts + (int64_t)blockIdx.x * (1 << kPartBits);
-  for (int k = threadIdx.x; k < nb; k += 256) cur[k] = in[k];
+  const int count = (int)(pc.hi - pc.lo);
+  for (int k = threadIdx.x; k < kB; k += 256) cur[k] = 0;
   __syncthreads();
-  for (int64_t x = pc.lo + threadIdx.x; x < pc.hi; x += 256) {
-    const uint2 r = staged[x];
-    sorted[atomicAdd(&cur[r.x - pc.bkey0], 1u)] = r;
+  for (int x = threadIdx.x; x < count; x += 256) atomicAdd(&cur[staged[pc.lo + x].x - pc.bkey0], 1u);
+  __syncthreads();
+  // exclusive scan of the chunk's histogram: kB / 256 buckets per thread, then over the threads
+  constexpr int kPer = kB > 256 ? kB / 256 : 1;
+  uint32_t loc[kPer], sum = 0;
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    const int j = threadIdx.x * kPer + e;
+    loc[e] = j < kB ? cur[j] : 0u;
+    sum += loc[e];
   }
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const uint32_t v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t at = part[threadIdx.x] - sum;
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    const int j = threadIdx.x * kPer + e;
+    if (j < kB) {
+      first[j] = at;
+      cur[j] = at;
+    }
+    at += loc[e];
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < count; x += 256) {
+    const uint2 r = staged[pc.lo + x];
+    buf[atomicAdd(&cur[r.x - pc.bkey0], 1u)] = r;
+  }
+  __syncthreads();
+  // buf is now in bucket order: slot x of bucket j goes to the chunk's position for j plus its rank
+  const uint32_t* base = counts + (int64_t)blockIdx.x * kB;
+  for (int x = threadIdx.x; x < count; x += 256) {
+    const uint2 r = buf[x];
+    const uint32_t j = r.x - (uint32_t)pc.bkey0;
+    sorted[base[j] + (x - first[j])] = r;
+  }
+  (void)nb;
 }
 
 template <class T>
