@@ -56,6 +56,7 @@
 
 #include "../../include/zkgpu.h"
 #include "zk_curve.hpp"
+#include "zk_curve_sliced.hpp"
 #include "zk_curve_wave.hpp"
 #include "zk_curve_xyzz.hpp"
 #include "zk_fields.hpp"
@@ -437,7 +438,8 @@ __device__ __forceinline__ bool part_chunk(const MsmJob* __restrict__ jobs, int 
 __global__ void __launch_bounds__(256) k_part_count(const MsmJob* __restrict__ jobs, int njobs,
                                                     const uint32_t* __restrict__ chunk_base, int64_t nparts,
                                                     const uint32_t* __restrict__ starts, const uint32_t* __restrict__ totals,
-                                                    const uint2* __restrict__ staged, uint32_t* __restrict__ counts) {
+                                                    const uint2* __restrict__ staged, uint32_t* __restrict__ counts,
+                                                    uint32_t* __restrict__ raw) {
   __shared__ uint32_t hist[1 << kPartBits];
   PartChunk pc;
   if (!part_chunk(jobs, njobs, chunk_base, nparts, starts, totals, blockIdx.x, &pc)) return;
@@ -447,7 +449,12 @@ __global__ void __launch_bounds__(256) k_part_count(const MsmJob* __restrict__ j
   for (int64_t x = pc.lo + threadIdx.x; x < pc.hi; x += 256) atomicAdd(&hist[staged[x].x - pc.bkey0], 1u);
   __syncthreads();
   uint32_t* out = counts + (int64_t)blockIdx.x * (1 << kPartBits);
-  for (int k = threadIdx.x; k < nb; k += 256) out[k] = hist[k];
+  uint32_t* keep = raw + (int64_t)blockIdx.x * (1 << kPartBits);
+  for (int k = threadIdx.x; k < (1 << kPartBits); k += 256) {
+    const uint32_t v = k < nb ? hist[k] : 0u;
+    out[k] = v;
+    keep[k] = v;  // the chunk's own histogram, for its placement
+  }
 }
 
 // Pass B, offsets: one workgroup per partition. Per bucket an exclusive scan down the partition's
@@ -506,7 +513,7 @@ __global__ void __launch_bounds__(256) k_part_place(const MsmJob* __restrict__ j
                                                     const uint32_t* __restrict__ chunk_base, int64_t nparts,
                                                     const uint32_t* __restrict__ starts, const uint32_t* __restrict__ totals,
                                                     const uint2* __restrict__ staged, const uint32_t* __restrict__ counts,
-                                                    uint2* __restrict__ sorted) {
+                                                    const uint32_t* __restrict__ raw, uint2* __restrict__ sorted) {
   constexpr int kB = 1 << kPartBits;
   __shared__ uint2 buf[kChunkB];
   __shared__ uint32_t first[kB];  // the bucket's first slot in buf
@@ -514,19 +521,16 @@ __global__ void __launch_bounds__(256) k_part_place(const MsmJob* __restrict__ j
   __shared__ uint32_t part[256];
   PartChunk pc;
   if (!part_chunk(jobs, njobs, chunk_base, nparts, starts, totals, blockIdx.x, &pc)) return;
-  const int nb = 1 << pc.pbits;
   const int count = (int)(pc.hi - pc.lo);
-  for (int k = threadIdx.x; k < kB; k += 256) cur[k] = 0;
-  __syncthreads();
-  for (int x = threadIdx.x; x < count; x += 256) atomicAdd(&cur[staged[pc.lo + x].x - pc.bkey0], 1u);
-  __syncthreads();
-  // exclusive scan of the chunk's histogram: kB / 256 buckets per thread, then over the threads
+  // exclusive scan of the chunk's histogram (kept by the count pass): kB / 256 buckets per thread,
+  // then over the threads
+  const uint32_t* hist = raw + (int64_t)blockIdx.x * kB;
   constexpr int kPer = kB > 256 ? kB / 256 : 1;
   uint32_t loc[kPer], sum = 0;
 #pragma unroll
   for (int e = 0; e < kPer; ++e) {
     const int j = threadIdx.x * kPer + e;
-    loc[e] = j < kB ? cur[j] : 0u;
+    loc[e] = j < kB ? hist[j] : 0u;
     sum += loc[e];
   }
   part[threadIdx.x] = sum;
@@ -560,7 +564,6 @@ __global__ void __launch_bounds__(256) k_part_place(const MsmJob* __restrict__ j
     const uint32_t j = r.x - (uint32_t)pc.bkey0;
     sorted[base[j] + (x - first[j])] = r;
   }
-  (void)nb;
 }
 
 template <class T>
@@ -972,15 +975,17 @@ __global__ void __launch_bounds__(16 * kWinRows) k_window_sums(const MsmJob* __r
   if (threadIdx.x == 0) wsum[g] = sh[0];
 }
 
-// One wavefront per job: Horner over the windows, sum_w 2^(cw) U_w (wave-parallel formulas).
+// One wavefront per job: Horner over the windows, sum_w 2^(cw) U_w, from the top window down; the
+// c doublings between windows with limb-sliced products (the chain's latency), the additions with
+// the wave-parallel formulas.
 __global__ void __launch_bounds__(64) k_final(const MsmJob* __restrict__ jobs, int njobs, const JacRec* __restrict__ wsum) {
   const int jb = blockIdx.x;
   if (jb >= njobs) return;
   const MsmJob& job = jobs[jb];
   const FqIlp f;
-  Jac acc = jac_identity(f);
-  for (int w = job.rw - 1; w >= 0; --w) {
-    for (int e = 0; e < job.c; ++e) acc = jac_dbl_wave(f, acc);
+  Jac acc = job.rw > 0 ? load_jac_rec(wsum + job.win_base + job.rw - 1) : jac_identity(f);
+  for (int w = job.rw - 2; w >= 0; --w) {
+    acc = jac_dbl_n_sliced<FqConst>(acc, job.c);
     acc = jac_add_wave(f, acc, load_jac_rec(wsum + job.win_base + w));
   }
   if (threadIdx.x == 0) store_jac(f, job.out, acc);
@@ -1062,7 +1067,7 @@ struct MsmPlan {
   void* mem = nullptr;
   MsmJob* d_jobs = nullptr;
   uint32_t *counts = nullptr, *totals = nullptr, *starts = nullptr, *m_dev = nullptr, *scan_ws = nullptr;
-  uint32_t *chunks = nullptr, *chunk_base = nullptr, *counts_b = nullptr;
+  uint32_t *chunks = nullptr, *chunk_base = nullptr, *counts_b = nullptr, *counts_raw = nullptr;
   uint2 *staged = nullptr, *sorted = nullptr;
   BucketRec* buckets = nullptr;
   RunSummary* runs[2] = {nullptr, nullptr};
@@ -1073,7 +1078,8 @@ static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Byte offsets of every buffer inside a plan's single device allocation.
 struct MsmLayout {
-  size_t jobs, counts, totals, starts, digits, m, staged, sorted, scan, chunks, chunk_base, counts_b, buckets, r0, r1, tot, run,
+  size_t jobs, counts, totals, starts, digits, m, staged, sorted, scan, chunks, chunk_base, counts_b, counts_raw, buckets, r0, r1,
+      tot, run,
       parts, wsum, total;
 };
 
@@ -1182,6 +1188,7 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   L->chunks = take(4 * (size_t)(pkeys + 1));
   L->chunk_base = take(4 * (size_t)(pkeys + 1));
   L->counts_b = take(4 * ((size_t)plan->max_chunks << kPartBits));
+  L->counts_raw = take(4 * ((size_t)plan->max_chunks << kPartBits));
   L->buckets = take(sizeof(BucketRec) * (size_t)keys);
   L->r0 = take(sizeof(RunSummary) * (size_t)max_slots);
   L->r1 = take(sizeof(RunSummary) * (size_t)max_slots1);
@@ -1219,6 +1226,7 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   plan->chunks = reinterpret_cast<uint32_t*>(base + L.chunks);
   plan->chunk_base = reinterpret_cast<uint32_t*>(base + L.chunk_base);
   plan->counts_b = reinterpret_cast<uint32_t*>(base + L.counts_b);
+  plan->counts_raw = reinterpret_cast<uint32_t*>(base + L.counts_raw);
   plan->scan_ws = reinterpret_cast<uint32_t*>(base + L.scan);
   plan->buckets = reinterpret_cast<BucketRec*>(base + L.buckets);
   plan->runs[0] = reinterpret_cast<RunSummary*>(base + L.r0);
@@ -1276,13 +1284,14 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
                              stream))
       return rc;
     hipLaunchKernelGGL(k_part_count, dim3((unsigned)plan->max_chunks), dim3(256), 0, stream, plan->d_jobs, nj, plan->chunk_base,
-                       plan->nparts, plan->starts, plan->totals, plan->staged, plan->counts_b);
+                       plan->nparts, plan->starts, plan->totals, plan->staged, plan->counts_b, plan->counts_raw);
     ZK_LAUNCH_CHECK("msm sort partition count");
     hipLaunchKernelGGL(k_part_columns, dim3((unsigned)plan->nparts), dim3(256), 0, stream, plan->d_jobs, nj, plan->chunk_base,
                        plan->nparts, plan->starts, plan->counts_b);
     ZK_LAUNCH_CHECK("msm sort partition columns");
     hipLaunchKernelGGL(k_part_place, dim3((unsigned)plan->max_chunks), dim3(256), 0, stream, plan->d_jobs, nj, plan->chunk_base,
-                       plan->nparts, plan->starts, plan->totals, plan->staged, plan->counts_b, plan->sorted);
+                       plan->nparts, plan->starts, plan->totals, plan->staged, plan->counts_b, plan->counts_raw,
+                       plan->sorted);
     ZK_LAUNCH_CHECK("msm sort partition place");
   }
   const int levels = (int)plan->level_max.size();
