@@ -1,4 +1,4 @@
-// Limb-sliced doubling chains: one limb of a coordinate per lane.
+// Limb-sliced point formulas for latency-bound chains: one limb of a coordinate per lane.
 //
 // A Horner pass over an MSM's windows is a chain of about a hundred doublings
 // on one point, so its time is the latency of one doubling times the chain.
@@ -109,14 +109,25 @@ __device__ __forceinline__ int32_t sl_fold(const SlLane& c, int64_t v) {
   return sl_carry(c, sl_carry64(c, (int64_t)n - (int64_t)q * (int64_t)c.p));
 }
 
-// Every row receives row 0's, 1's and 2's value.
-__device__ __forceinline__ void sl_rows(int32_t x, int32_t& r0, int32_t& r1, int32_t& r2) {
+// Every row receives row 0's, 1's, 2's and 3's value.
+__device__ __forceinline__ void sl_rows(int32_t x, int32_t& r0, int32_t& r1, int32_t& r2, int32_t& r3) {
   const auto s = __builtin_amdgcn_permlane16_swap((uint32_t)x, (uint32_t)x, false, false);  // (x0 x0 x2 x2), (x1 x1 x3 x3)
   const auto e = __builtin_amdgcn_permlane32_swap(s[0], s[0], false, false);               // (x0 x0 x0 x0), (x2 x2 x2 x2)
   const auto o = __builtin_amdgcn_permlane32_swap(s[1], s[1], false, false);               // (x1 x1 x1 x1), (x3 x3 x3 x3)
   r0 = (int32_t)e[0];
   r1 = (int32_t)o[0];
   r2 = (int32_t)e[1];
+  r3 = (int32_t)o[1];
+}
+
+__device__ __forceinline__ void sl_rows(int32_t x, int32_t& r0, int32_t& r1, int32_t& r2) {
+  int32_t r3;
+  sl_rows(x, r0, r1, r2, r3);
+}
+
+// The operand of this lane's row: x0, x1, x2 or x3 by row.
+__device__ __forceinline__ int32_t sl_pick(const SlLane& c, int32_t x0, int32_t x1, int32_t x2, int32_t x3) {
+  return c.row == 0 ? x0 : (c.row == 1 ? x1 : (c.row == 2 ? x2 : x3));
 }
 
 struct SlJac {
@@ -181,6 +192,56 @@ __device__ __forceinline__ Jac sl_to_jac(const SlLane& c, const SlJac& s) {
   r.x = sl_to_fe<F>(c, s.x);
   r.y = sl_to_fe<F>(c, s.y);
   r.z = sl_to_fe<F>(c, s.z);
+  return r;
+}
+
+// Whether v (|v| < 2.1p) is 0 mod p, in every lane (all rows hold the same value).
+template <class F>
+__device__ __forceinline__ bool sl_is_zero(const SlLane& c, int32_t v) {
+  return fe_is_zero_lazy(F{}, sl_to_fe<F>(c, v));
+}
+
+// P + Q, add-2007-bl (U1 = X1 Z2^2, U2 = X2 Z1^2, S1 = Y1 Z2^3, S2 = Y2 Z1^3, H = U2 - U1,
+// I = (2H)^2, J = H I, r = 2(S2 - S1), V = U1 I, X3 = r^2 - J - 2V, Y3 = r (V - X3) - 2 S1 J,
+// Z3 = 2 Z1 Z2 H) in five levels of at most four products, one per row:
+//   {Z1Z1, Z2Z2, Y1 Z2, Y2 Z1}, {U1, U2, S1, S2}, {I, r^2, Z1 Z2}, {J, V, Z1 Z2 H},
+//   {r (3V + J - r^2), S1 J}.
+// Inputs below 2.02p in magnitude: the products of the first two levels lie in (-0.03p, 1.03p),
+// |2H|, |r| < 2.06p, the products after them in (-0.05p, 1.05p), V - X3 = 3V + J - r^2 below 4.1p;
+// outputs below 2.02p. P = Q doubles, P = -Q gives the identity (Z = 0), as zk_curve.hpp does.
+template <class F>
+__device__ __forceinline__ SlJac sl_add(const SlLane& c, const SlJac& p, const SlJac& q) {
+  if (sl_is_zero<F>(c, p.z)) return q;
+  if (sl_is_zero<F>(c, q.z)) return p;
+  const int32_t r1 = sl_mul<F>(c, sl_pick(c, p.z, q.z, p.y, q.y), sl_pick(c, p.z, q.z, q.z, p.z));
+  int32_t z1z1, z2z2, y1z2, y2z1;
+  sl_rows(r1, z1z1, z2z2, y1z2, y2z1);
+  const int32_t r2 = sl_mul<F>(c, sl_pick(c, p.x, q.x, y1z2, y2z1), sl_pick(c, z2z2, z1z1, z2z2, z1z1));
+  int32_t u1, u2, s1, s2;
+  sl_rows(r2, u1, u2, s1, s2);
+  const int32_t h = sl_carry(c, u2 - u1);
+  const int32_t rr = sl_carry(c, 2 * (s2 - s1));
+  if (sl_is_zero<F>(c, h)) {
+    if (sl_is_zero<F>(c, rr)) return sl_dbl<F>(c, p);
+    SlJac id = p;
+    id.z = 0;
+    return id;
+  }
+  const int32_t h2 = sl_carry(c, 2 * h);
+  const int32_t r3 = sl_mul<F>(c, sl_pick(c, h2, rr, p.z, p.z), sl_pick(c, h2, rr, q.z, q.z));
+  int32_t i, rsq, z12;
+  sl_rows(r3, i, rsq, z12);
+  const int32_t r4 = sl_mul<F>(c, sl_pick(c, h, u1, z12, z12), sl_pick(c, i, i, h, h));
+  int32_t j, v, zh;
+  sl_rows(r4, j, v, zh);
+  const int32_t e = sl_carry(c, sl_carry64(c, 3 * (int64_t)v + (int64_t)j - (int64_t)rsq));
+  const int32_t r5 = sl_mul<F>(c, c.row == 0 ? rr : s1, c.row == 0 ? e : j);
+  int32_t m1, m2, m3;  // rows 1..3 hold the same product
+  sl_rows(r5, m1, m2, m3);
+  SlJac r;
+  r.x = sl_fold<F>(c, (int64_t)rsq - (int64_t)j - 2 * (int64_t)v);
+  r.y = sl_fold<F>(c, (int64_t)m1 - 2 * (int64_t)m2);
+  r.z = sl_fold<F>(c, 2 * (int64_t)zh);
   return r;
 }
 

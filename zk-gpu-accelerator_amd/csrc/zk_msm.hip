@@ -915,80 +915,90 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
   }
   store_jac_rec(&sh[threadIdx.x], acc);
   __syncthreads();
-  // a tree over the 256 partial sums: one addition per thread while the level has more additions
-  // than the workgroup has 16-lane rows, then one per row (wave-parallel formulas)
-  for (int off = 128; off > 0; off >>= 1) {
-    if (off > 16) {
-      if ((int)threadIdx.x < off) {
-        Jac a = load_jac_rec(&sh[threadIdx.x]);
-        a = jac_add(f, a, load_jac_rec(&sh[threadIdx.x + off]));
-        store_jac_rec(&sh[threadIdx.x], a);
-      }
-    } else {
-      const int row = (int)threadIdx.x >> 4;
-      if (row < off) {
-        const Jac a = jac_add_wave(f, load_jac_rec(&sh[row]), load_jac_rec(&sh[row + off]));
-        if ((threadIdx.x & 15) == 0) store_jac_rec(&sh[row], off == 1 ? jac_canonical(f, a) : a);
-      }
+  // a tree over the 256 partial sums: one addition per thread down to 16 sums, then the last
+  // levels on the four wavefronts with the limb-sliced formulas (an addition in about a third of
+  // one lane's time)
+  for (int off = 128; off > 8; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      Jac a = load_jac_rec(&sh[threadIdx.x]);
+      a = jac_add(f, a, load_jac_rec(&sh[threadIdx.x + off]));
+      store_jac_rec(&sh[threadIdx.x], a);
+    }
+    __syncthreads();
+  }
+  const SlLane c = sl_lane<FqConst>();
+  const int wave = (int)threadIdx.x >> 6;
+  for (int off = 8; off > 0; off >>= 1) {
+    for (int i = wave; i < off; i += 4) {
+      const SlJac a = sl_add<FqConst>(c, sl_from_jac(load_jac_rec(&sh[i])), sl_from_jac(load_jac_rec(&sh[i + off])));
+      const Jac r = sl_to_jac<FqConst>(c, a);
+      if ((threadIdx.x & 63) == 0) store_jac_rec(&sh[i], off == 1 ? jac_canonical(f, r) : r);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) parts[blockIdx.x] = sh[0];
 }
 
-// One workgroup of 16 rows per (job, window): U = sum T_s + sum_j 2^(j + log_seg) Q_j. Row r takes
-// the terms r, r + 16, ...: a term is the sum of its reduction row's chunk partials; term
-// j < log_segs is Q_j doubled j + log_seg times, the last two are the halves of sum T_s;
-// the rows' sums then meet in a tree. The doublings of different terms run side by side (a Horner
-// pass over j would chain all of them with the additions), each row with the wave-parallel formulas.
-constexpr int kWinRows = 16;
+// One workgroup of 16 wavefronts per (job, window): U = sum T_s + sum_j 2^(j + log_seg) Q_j. Wavefront
+// w takes the terms w, w + 16, ...: a term is the sum of its reduction row's chunk partials; term
+// j < log_segs is Q_j doubled j + log_seg times, the last two are the halves of sum T_s; the
+// wavefronts' sums then meet in a tree. The doublings of different terms run side by side (a Horner
+// pass over j would chain all of them with the additions), each on one wavefront with the
+// limb-sliced formulas.
+constexpr int kWinWaves = 16;
 
-__global__ void __launch_bounds__(16 * kWinRows) k_window_sums(const MsmJob* __restrict__ jobs, int njobs,
-                                                               int64_t total_windows, const JacRec* __restrict__ parts,
-                                                               JacRec* __restrict__ wsum) {
-  __shared__ JacRec sh[kWinRows];
+__global__ void __launch_bounds__(64 * kWinWaves) k_window_sums(const MsmJob* __restrict__ jobs, int njobs,
+                                                                int64_t total_windows, const JacRec* __restrict__ parts,
+                                                                JacRec* __restrict__ wsum) {
+  __shared__ JacRec sh[kWinWaves];
   const int64_t g = blockIdx.x;
   if (g >= total_windows) return;
   const MsmJob& job = jobs[find_job<&MsmJob::win_base>(jobs, njobs, g)];
   const int64_t win = g - job.win_base;
   const int per = red_rows(job);
   const JacRec* p = parts + job.part_base + win * per * job.chunks;
-  const FqIlp f;
-  const int row = (int)threadIdx.x >> 4;
-  Jac acc = jac_identity(f);
-  for (int t = row; t < per; t += kWinRows) {
-    Jac d = load_jac_rec(p + (int64_t)t * job.chunks);
-    for (int k = 1; k < job.chunks; ++k) d = jac_add_wave(f, d, load_jac_rec(p + (int64_t)t * job.chunks + k));
+  const FqConst f;
+  const SlLane c = sl_lane<FqConst>();
+  const int wave = (int)threadIdx.x >> 6;
+  SlJac acc = sl_from_jac(jac_identity(f));
+  for (int t = wave; t < per; t += kWinWaves) {
+    SlJac d = sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks));
+    for (int k = 1; k < job.chunks; ++k) d = sl_add<FqConst>(c, d, sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks + k)));
     const int dbl = t < job.log_segs ? t + job.log_seg : 0;
-    for (int e = 0; e < dbl; ++e) d = jac_dbl_wave(f, d);
-    acc = jac_add_wave(f, acc, d);
+    for (int e = 0; e < dbl; ++e) d = sl_dbl<FqConst>(c, d);
+    acc = sl_add<FqConst>(c, acc, d);
   }
-  if ((threadIdx.x & 15) == 0) store_jac_rec(&sh[row], acc);
-  for (int d = 1; d < kWinRows; d <<= 1) {
+  const bool lead = (threadIdx.x & 63) == 0;
+  const Jac mine = sl_to_jac<FqConst>(c, acc);
+  if (lead) store_jac_rec(&sh[wave], mine);
+  for (int d = 1; d < kWinWaves; d <<= 1) {
     __syncthreads();
-    if (row < kWinRows / (2 * d)) {
-      const Jac a = jac_add_wave(f, load_jac_rec(&sh[2 * d * row]), load_jac_rec(&sh[2 * d * row + d]));
-      if ((threadIdx.x & 15) == 0) store_jac_rec(&sh[2 * d * row], a);
+    if ((wave & (2 * d - 1)) == 0) {
+      const SlJac a = sl_add<FqConst>(c, sl_from_jac(load_jac_rec(&sh[wave])), sl_from_jac(load_jac_rec(&sh[wave + d])));
+      const Jac r = sl_to_jac<FqConst>(c, a);
+      if (lead) store_jac_rec(&sh[wave], r);
     }
   }
   __syncthreads();
   if (threadIdx.x == 0) wsum[g] = sh[0];
 }
 
-// One wavefront per job: Horner over the windows, sum_w 2^(cw) U_w, from the top window down; the
-// c doublings between windows with limb-sliced products (the chain's latency), the additions with
-// the wave-parallel formulas.
+// One wavefront per job: Horner over the windows, sum_w 2^(cw) U_w, from the top window down, with
+// the limb-sliced formulas (the chain's latency is the doubling's).
 __global__ void __launch_bounds__(64) k_final(const MsmJob* __restrict__ jobs, int njobs, const JacRec* __restrict__ wsum) {
   const int jb = blockIdx.x;
   if (jb >= njobs) return;
   const MsmJob& job = jobs[jb];
-  const FqIlp f;
-  Jac acc = job.rw > 0 ? load_jac_rec(wsum + job.win_base + job.rw - 1) : jac_identity(f);
+  const FqConst f;
+  const SlLane c = sl_lane<FqConst>();
+  SlJac acc = sl_from_jac(job.rw > 0 ? load_jac_rec(wsum + job.win_base + job.rw - 1) : jac_identity(f));
   for (int w = job.rw - 2; w >= 0; --w) {
-    acc = jac_dbl_n_sliced<FqConst>(acc, job.c);
-    acc = jac_add_wave(f, acc, load_jac_rec(wsum + job.win_base + w));
+    for (int e = 0; e < job.c; ++e) acc = sl_dbl<FqConst>(c, acc);
+    acc = sl_add<FqConst>(c, acc, sl_from_jac(load_jac_rec(wsum + job.win_base + w)));
   }
-  if (threadIdx.x == 0) store_jac(f, job.out, acc);
+  Jac r = sl_to_jac<FqConst>(c, acc);
+  if (fe_is_zero_lazy(f, r.z)) r = jac_identity(f);
+  if (threadIdx.x == 0) store_jac(f, job.out, r);
 }
 
 // Window table of an SRS: out[w * n + i] = 2^(c w) P_i as an affine record, w < windows (one
@@ -1332,7 +1342,7 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
     hipLaunchKernelGGL(k_reduce_bits, dim3((unsigned)plan->parts), dim3(256), 0, stream, plan->d_jobs, nj, plan->tot,
                        plan->run, plan->parts_buf);
     ZK_LAUNCH_CHECK("msm reduce bits");
-    hipLaunchKernelGGL(k_window_sums, dim3((unsigned)plan->windows_total), dim3(16 * kWinRows), 0, stream, plan->d_jobs, nj,
+    hipLaunchKernelGGL(k_window_sums, dim3((unsigned)plan->windows_total), dim3(64 * kWinWaves), 0, stream, plan->d_jobs, nj,
                        plan->windows_total, plan->parts_buf, plan->wsum);
     ZK_LAUNCH_CHECK("msm window sums");
   }
