@@ -176,6 +176,12 @@ int zk_msm_get_glv(int* on);
  * 0 = Jacobian madd-2007-bl with serial column products. No counterpart in the reference. */
 int zk_msm_set_accumulate(int variant);
 int zk_msm_get_accumulate(int* variant);
+/* log2 of the bucket-reduction segment length L of plans created afterwards (picked by the
+ * autotuner): each thread forms the running sums of L consecutive buckets, the segments' sums
+ * then meet in log2(buckets / L) bit rows; 0 = automatic (L = 4). The reference's segments are a
+ * fixed length and their sums are scaled one by one (msm_gpu.py:196-223). */
+int zk_msm_set_segment(int log_seg);
+int zk_msm_get_segment(int* log_seg);
 /* A device-resident basis: n affine mont points converted once into the kernels' internal
  * layout and kept in HBM (the shared SRS of batched and served MSMs; no counterpart in the
  * reference, which converts its points on every call, msm_gpu.py:348). */

@@ -208,6 +208,31 @@ def test_accumulation_variants(variant):
         gpu_backend.call("zk_msm_set_accumulate", old.value)
 
 
+@pytest.mark.parametrize("log_seg", [1, 3, 4])
+def test_reduction_segment_lengths(log_seg):
+    """The bucket reduction with other segment lengths (fewer or more bit rows per window), over
+    windows small enough that a segment covers a window's buckets, and over tuned ones."""
+    import ctypes
+
+    n = 3000
+    keys = workloads.srs_keys(n, 0x5EED10)
+    scalars = workloads.witness_scalars(n, 0x5EED11)
+    scalars[:3] = [R - 1, 2**127 - 1, 0]
+    pts = workloads.srs_points(ints_to_limbs(keys))
+    e = workloads.closed_form_scalar(keys, scalars)
+    old = ctypes.c_int()
+    gpu_backend.call("zk_msm_get_segment", ctypes.byref(old))
+    gpu_backend.call("zk_msm_set_segment", log_seg)
+    try:
+        for w in (0, 3, 6, 12):
+            out = gpu_backend.empty((3, 4))
+            msm_gpu.msm_g1_jacobian(pts, gpu_backend.asarray(ints_to_limbs(scalars)), out, window_size=w)
+            got = msm_gpu._result(out)
+            assert (got.is_identity() and e == 0) or got.affine() == (G * e).affine(), (log_seg, w)
+    finally:
+        gpu_backend.call("zk_msm_set_segment", old.value)
+
+
 @pytest.mark.parametrize("chunk", [0, 2, 5])
 def test_one_bucket_spans_every_range(chunk):
     """Scalars that send (almost) every entry to one bucket: the bucket's run crosses every range

@@ -136,9 +136,10 @@ struct SlJac {
 
 // 2P, dbl-2009-l in the three product levels of jac_dbl_wave ({A, B, YZ}, {C, T = (X + B)^2, A^2},
 // {P = A w}, w = 2T - 2A - 2C - 3A^2; X3 = 9A^2 - 4T + 4A + 4C, Y3 = 9P - 8C, Z3 = 2YZ), one row per
-// product. Inputs and outputs below 2.02p in magnitude: then A, B, C, A^2, YZ are below 1.03p,
-// T (of |X + B| < 3.1p) below 1.06p, |w| < 7.1p, P below 1.05p, and the raw X3, Y3, Z3 below 17.2p.
-// Z = 0 mod p stays so (the identity doubles to itself).
+// product. X, Y below 2.02p and Z below 2.1p in magnitude, in and out: then A, B, C, A^2, YZ are
+// below 1.03p, T (of |X + B| < 3.1p) below 1.06p, |w| < 7.1p, P below 1.05p, the raw X3, Y3 below
+// 17.2p (folded), and 2YZ below 2.06p (only normalized). Z = 0 mod p stays so (the identity
+// doubles to itself).
 template <class F>
 __device__ __forceinline__ SlJac sl_dbl(const SlLane& c, const SlJac& q) {
   const int32_t r1 = sl_mul<F>(c, c.row == 0 ? q.x : q.y, c.row == 0 ? q.x : (c.row == 1 ? q.y : q.z));
@@ -155,7 +156,7 @@ __device__ __forceinline__ SlJac sl_dbl(const SlLane& c, const SlJac& q) {
   SlJac r;
   r.x = sl_fold<F>(c, 9 * (int64_t)a2 - 4 * (int64_t)t + 4 * (int64_t)a + 4 * (int64_t)cc);
   r.y = sl_fold<F>(c, 9 * (int64_t)pa - 8 * (int64_t)cc);
-  r.z = sl_fold<F>(c, 2 * (int64_t)yz);
+  r.z = sl_carry(c, 2 * yz);
   return r;
 }
 
@@ -168,7 +169,7 @@ __device__ __forceinline__ int32_t sl_from_fe(const Fe& x) {
   return (int32_t)v;
 }
 
-// A sliced value (|v| < 2.02p) back to a coordinate below 2p with limbs below 2^29, in every lane:
+// A sliced value (|v| < 2.1p) back to a coordinate below 2p with limbs below 2^29, in every lane:
 // v + 3p is positive and below 2^256, its limbs gathered and reduced as lc_finish does.
 template <class F>
 __device__ __forceinline__ Fe sl_to_fe(const SlLane& c, int32_t v) {
@@ -206,9 +207,10 @@ __device__ __forceinline__ bool sl_is_zero(const SlLane& c, int32_t v) {
 // Z3 = 2 Z1 Z2 H) in five levels of at most four products, one per row:
 //   {Z1Z1, Z2Z2, Y1 Z2, Y2 Z1}, {U1, U2, S1, S2}, {I, r^2, Z1 Z2}, {J, V, Z1 Z2 H},
 //   {r (3V + J - r^2), S1 J}.
-// Inputs below 2.02p in magnitude: the products of the first two levels lie in (-0.03p, 1.03p),
-// |2H|, |r| < 2.06p, the products after them in (-0.05p, 1.05p), V - X3 = 3V + J - r^2 below 4.1p;
-// outputs below 2.02p. P = Q doubles, P = -Q gives the identity (Z = 0), as zk_curve.hpp does.
+// Inputs X, Y below 2.02p and Z below 2.1p in magnitude: the products of the first two levels lie
+// in (-0.03p, 1.03p), |2H|, |r| < 2.06p, the products after them in (-0.05p, 1.05p), V - X3 =
+// 3V + J - r^2 below 4.1p; outputs below 2.02p. P = Q doubles, P = -Q gives the identity (Z = 0),
+// as zk_curve.hpp does.
 template <class F>
 __device__ __forceinline__ SlJac sl_add(const SlLane& c, const SlJac& p, const SlJac& q) {
   if (sl_is_zero<F>(c, p.z)) return q;

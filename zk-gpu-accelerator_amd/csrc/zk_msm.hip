@@ -939,39 +939,24 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
   if (threadIdx.x == 0) parts[blockIdx.x] = sh[0];
 }
 
-// One workgroup of 16 wavefronts per (job, window): U = sum T_s + sum_j 2^(j + log_seg) Q_j. Wavefront
-// w takes the terms w, w + 16, ...: a term is the sum of its reduction row's chunk partials; term
-// j < log_segs is Q_j doubled j + log_seg times, the last two are the halves of sum T_s; the
-// wavefronts' sums then meet in a tree. The doublings of different terms run side by side (a Horner
-// pass over j would chain all of them with the additions), each on one wavefront with the
-// limb-sliced formulas.
-constexpr int kWinWaves = 16;
+// Window sums U = sum T_s + sum_j 2^(j + log_seg) Q_j, in two launches. k_window_terms: four
+// workgroups of four wavefronts per (job, window), one wavefront per SIMD (the limb-sliced formulas
+// keep a SIMD's issue busy with one wavefront, so a second one there would only wait). Wavefront
+// slot i of the window takes the terms i, i + 16, ...: a term is the sum of its reduction row's
+// chunk partials; term j < log_segs is Q_j doubled j + log_seg times, the last two are the halves
+// of sum T_s. The doublings of different terms run side by side (a Horner pass over j would chain
+// all of them with the additions); each workgroup sums its wavefronts' terms, and k_window_total
+// sums a window's four partials.
+constexpr int kWinGroups = 4;
+constexpr int kWinWaves = 4;
 
-__global__ void __launch_bounds__(64 * kWinWaves) k_window_sums(const MsmJob* __restrict__ jobs, int njobs,
-                                                                int64_t total_windows, const JacRec* __restrict__ parts,
-                                                                JacRec* __restrict__ wsum) {
-  __shared__ JacRec sh[kWinWaves];
-  const int64_t g = blockIdx.x;
-  if (g >= total_windows) return;
-  const MsmJob& job = jobs[find_job<&MsmJob::win_base>(jobs, njobs, g)];
-  const int64_t win = g - job.win_base;
-  const int per = red_rows(job);
-  const JacRec* p = parts + job.part_base + win * per * job.chunks;
-  const FqConst f;
-  const SlLane c = sl_lane<FqConst>();
+template <int kWaves>
+__device__ __forceinline__ SlJac sl_workgroup_sum(JacRec* sh, const SlLane& c, const SlJac& mine) {
   const int wave = (int)threadIdx.x >> 6;
-  SlJac acc = sl_from_jac(jac_identity(f));
-  for (int t = wave; t < per; t += kWinWaves) {
-    SlJac d = sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks));
-    for (int k = 1; k < job.chunks; ++k) d = sl_add<FqConst>(c, d, sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks + k)));
-    const int dbl = t < job.log_segs ? t + job.log_seg : 0;
-    for (int e = 0; e < dbl; ++e) d = sl_dbl<FqConst>(c, d);
-    acc = sl_add<FqConst>(c, acc, d);
-  }
   const bool lead = (threadIdx.x & 63) == 0;
-  const Jac mine = sl_to_jac<FqConst>(c, acc);
-  if (lead) store_jac_rec(&sh[wave], mine);
-  for (int d = 1; d < kWinWaves; d <<= 1) {
+  const Jac v = sl_to_jac<FqConst>(c, mine);
+  if (lead) store_jac_rec(&sh[wave], v);
+  for (int d = 1; d < kWaves; d <<= 1) {
     __syncthreads();
     if ((wave & (2 * d - 1)) == 0) {
       const SlJac a = sl_add<FqConst>(c, sl_from_jac(load_jac_rec(&sh[wave])), sl_from_jac(load_jac_rec(&sh[wave + d])));
@@ -980,7 +965,45 @@ __global__ void __launch_bounds__(64 * kWinWaves) k_window_sums(const MsmJob* __
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) wsum[g] = sh[0];
+  return sl_from_jac(load_jac_rec(&sh[0]));
+}
+
+__global__ void __launch_bounds__(64 * kWinWaves) k_window_terms(const MsmJob* __restrict__ jobs, int njobs,
+                                                                 int64_t total_windows, const JacRec* __restrict__ parts,
+                                                                 JacRec* __restrict__ wparts) {
+  __shared__ JacRec sh[kWinWaves];
+  const int64_t g = blockIdx.x / kWinGroups;
+  const int group = (int)(blockIdx.x % kWinGroups);
+  if (g >= total_windows) return;
+  const MsmJob& job = jobs[find_job<&MsmJob::win_base>(jobs, njobs, g)];
+  const int64_t win = g - job.win_base;
+  const int per = red_rows(job);
+  const JacRec* p = parts + job.part_base + win * per * job.chunks;
+  const FqConst f;
+  const SlLane c = sl_lane<FqConst>();
+  SlJac acc = sl_from_jac(jac_identity(f));
+  for (int t = group * kWinWaves + ((int)threadIdx.x >> 6); t < per; t += kWinGroups * kWinWaves) {
+    SlJac d = sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks));
+    for (int k = 1; k < job.chunks; ++k) d = sl_add<FqConst>(c, d, sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks + k)));
+    const int dbl = t < job.log_segs ? t + job.log_seg : 0;
+    for (int e = 0; e < dbl; ++e) d = sl_dbl<FqConst>(c, d);
+    acc = sl_add<FqConst>(c, acc, d);
+  }
+  acc = sl_workgroup_sum<kWinWaves>(sh, c, acc);
+  const Jac r = sl_to_jac<FqConst>(c, acc);
+  if (threadIdx.x == 0) store_jac_rec(wparts + blockIdx.x, r);
+}
+
+// One workgroup per (job, window): the sum of the window's kWinGroups partials.
+__global__ void __launch_bounds__(64 * kWinGroups) k_window_total(int64_t total_windows, const JacRec* __restrict__ wparts,
+                                                                  JacRec* __restrict__ wsum) {
+  __shared__ JacRec sh[kWinGroups];
+  const int64_t g = blockIdx.x;
+  if (g >= total_windows) return;
+  const SlLane c = sl_lane<FqConst>();
+  const SlJac mine = sl_from_jac(load_jac_rec(wparts + g * kWinGroups + ((int)threadIdx.x >> 6)));
+  const Jac r = sl_to_jac<FqConst>(c, sl_workgroup_sum<kWinGroups>(sh, c, mine));
+  if (threadIdx.x == 0) store_jac_rec(wsum + g, r);
 }
 
 // One wavefront per job: Horner over the windows, sum_w 2^(cw) U_w, from the top window down, with
@@ -1024,6 +1047,9 @@ __global__ void __launch_bounds__(256) k_srs_window_table(const AffRec* __restri
 
 // Whether new plans split scalars with the GLV endomorphism (zk_msm_set_glv).
 static int g_glv = 1;
+
+// log2 of the reduction's segment length for new plans (zk_msm_set_segment); 0: min(2, c - 1).
+static int g_log_seg = 0;
 
 // The bucket accumulation kernel new plans use (zk_msm_set_accumulate): bit 0 picks the
 // low-latency product schedule, bit 1 XYZZ coordinates.
@@ -1081,7 +1107,7 @@ struct MsmPlan {
   uint2 *staged = nullptr, *sorted = nullptr;
   BucketRec* buckets = nullptr;
   RunSummary* runs[2] = {nullptr, nullptr};
-  JacRec *tot = nullptr, *run = nullptr, *parts_buf = nullptr, *wsum = nullptr;
+  JacRec *tot = nullptr, *run = nullptr, *parts_buf = nullptr, *wparts = nullptr, *wsum = nullptr;
 };
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1090,7 +1116,7 @@ static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 struct MsmLayout {
   size_t jobs, counts, totals, starts, digits, m, staged, sorted, scan, chunks, chunk_base, counts_b, counts_raw, buckets, r0, r1,
       tot, run,
-      parts, wsum, total;
+      parts, wparts, wsum, total;
 };
 
 // jobs: scalars, n, c (0 = default) and out filled in by the caller; the rest is planned here,
@@ -1107,7 +1133,7 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
     j.nv = j.glv ? 2 * j.n : j.n;
     j.windows = (j.glv ? Glv::kBits : 254) / j.c + 1;
     j.rw = j.table ? 1 : j.windows;
-    j.log_seg = std::min(2, j.c - 1);
+    j.log_seg = std::min(g_log_seg > 0 ? g_log_seg : 2, j.c - 1);
     j.log_segs = (j.c - 1) - j.log_seg;
     // chunks per reduction row: a row adds half the segments, or (the first T half) ceil(segs / 2)
     j.chunks = (int)((((1LL << j.log_segs) + 1) / 2 + kRedChunk - 1) / kRedChunk);
@@ -1205,6 +1231,7 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   L->tot = take(sizeof(JacRec) * (size_t)segs);
   L->run = take(sizeof(JacRec) * (size_t)segs);
   L->parts = take(sizeof(JacRec) * (size_t)parts);
+  L->wparts = take(sizeof(JacRec) * (size_t)wins * kWinGroups);
   L->wsum = take(sizeof(JacRec) * (size_t)wins);
   L->total = off;
   return 0;
@@ -1244,6 +1271,7 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   plan->tot = reinterpret_cast<JacRec*>(base + L.tot);
   plan->run = reinterpret_cast<JacRec*>(base + L.run);
   plan->parts_buf = reinterpret_cast<JacRec*>(base + L.parts);
+  plan->wparts = reinterpret_cast<JacRec*>(base + L.wparts);
   plan->wsum = reinterpret_cast<JacRec*>(base + L.wsum);
   // Stream-ordered upload from the plan's own job table, which lives as long as the plan.
   if (!plan->jobs.empty()) {
@@ -1342,8 +1370,11 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
     hipLaunchKernelGGL(k_reduce_bits, dim3((unsigned)plan->parts), dim3(256), 0, stream, plan->d_jobs, nj, plan->tot,
                        plan->run, plan->parts_buf);
     ZK_LAUNCH_CHECK("msm reduce bits");
-    hipLaunchKernelGGL(k_window_sums, dim3((unsigned)plan->windows_total), dim3(64 * kWinWaves), 0, stream, plan->d_jobs, nj,
-                       plan->windows_total, plan->parts_buf, plan->wsum);
+    hipLaunchKernelGGL(k_window_terms, dim3((unsigned)(plan->windows_total * kWinGroups)), dim3(64 * kWinWaves), 0, stream,
+                       plan->d_jobs, nj, plan->windows_total, plan->parts_buf, plan->wparts);
+    ZK_LAUNCH_CHECK("msm window terms");
+    hipLaunchKernelGGL(k_window_total, dim3((unsigned)plan->windows_total), dim3(64 * kWinGroups), 0, stream,
+                       plan->windows_total, plan->wparts, plan->wsum);
     ZK_LAUNCH_CHECK("msm window sums");
   }
   Prof prof("msm_final", stream);
@@ -1413,6 +1444,17 @@ int zk_msm_set_accumulate(int variant) {
 
 int zk_msm_get_accumulate(int* variant) {
   *variant = g_acc_variant;
+  return 0;
+}
+
+int zk_msm_set_segment(int log_seg) {
+  if (log_seg < 0 || log_seg > 6) return fail("segment length must be 2^0..2^6 (0: automatic)");
+  g_log_seg = log_seg;
+  return 0;
+}
+
+int zk_msm_get_segment(int* log_seg) {
+  *log_seg = g_log_seg;
   return 0;
 }
 

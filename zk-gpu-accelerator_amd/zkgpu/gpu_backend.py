@@ -90,6 +90,8 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_msm_get_glv": [ctypes.POINTER(_i32)],
     "zk_msm_set_accumulate": [_i32],
     "zk_msm_get_accumulate": [ctypes.POINTER(_i32)],
+    "zk_msm_set_segment": [_i32],
+    "zk_msm_get_segment": [ctypes.POINTER(_i32)],
     "zk_prof_enable": [_i32],
     "zk_prof_read": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), _i64p, _i32],
     "zk_field_register": [_u64p, ctypes.POINTER(_i32)],
