@@ -872,7 +872,11 @@ __global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ 
   store_jac_rec(run + g, r);
 }
 
-constexpr int kRedPerThread = 4;                      // segments each thread adds serially
+// Segments each thread adds serially. The rows' additions are a latency chain (the serial sums,
+// then a tree), so the kernel wants about one wavefront per SIMD rather than more threads: with
+// 8, a 2^20-point MSM (16-bit windows: 8 windows x 15 rows x 2 chunks) runs 240 workgroups of
+// four wavefronts.
+constexpr int kRedPerThread = 8;
 constexpr int kRedChunk = 256 * kRedPerThread;          // segments per workgroup
 
 // Rows of a window's reduction: Q_j for j < log_segs (the segments whose index has bit j set,
@@ -880,7 +884,7 @@ constexpr int kRedChunk = 256 * kRedPerThread;          // segments per workgrou
 // row has the same number of additions.
 __device__ __forceinline__ int red_rows(const MsmJob& job) { return job.log_segs + 2; }
 
-// One workgroup per (job, window, row, chunk): 256 threads x 4 selected segments, then a tree.
+// One workgroup per (job, window, row, chunk): 256 threads x kRedPerThread selected segments, then a tree.
 __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ jobs, int njobs,
                                                      const JacRec* __restrict__ tot, const JacRec* __restrict__ run,
                                                      JacRec* __restrict__ parts) {
@@ -901,9 +905,15 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
     // neighbouring lanes never skip additions the wavefront executes anyway
     const JacRec* src = run + job.seg_base + win * segs;
     const int64_t hi = min(segs >> 1, lo + (int64_t)kRedChunk), low = (1LL << j) - 1;
-    for (int64_t k = lo + threadIdx.x; k < hi; k += 256) {
-      const int64_t sg = ((k >> j) << (j + 1)) + (1LL << j) + (k & low);
-      acc = jac_add(f, acc, load_jac_rec(src + sg));
+    auto seg = [&](int64_t k) { return ((k >> j) << (j + 1)) + (1LL << j) + (k & low); };
+    int64_t k = lo + threadIdx.x;
+    if (k < hi) {
+      Jac next = load_jac_rec(src + seg(k));  // the next segment's load overlaps this addition
+      for (; k < hi; k += 256) {
+        const Jac cur = next;
+        if (k + 256 < hi) next = load_jac_rec(src + seg(k + 256));
+        acc = jac_add(f, acc, cur);
+      }
     }
   } else {
     // the T sum over one half of the segments
@@ -911,7 +921,15 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
     const int64_t base = j == job.log_segs ? 0 : first, count = j == job.log_segs ? first : segs - first;
     const JacRec* src = tot + job.seg_base + win * segs + base;
     const int64_t hi = min(count, lo + (int64_t)kRedChunk);
-    for (int64_t k = lo + threadIdx.x; k < hi; k += 256) acc = jac_add(f, acc, load_jac_rec(src + k));
+    int64_t k = lo + threadIdx.x;
+    if (k < hi) {
+      Jac next = load_jac_rec(src + k);
+      for (; k < hi; k += 256) {
+        const Jac cur = next;
+        if (k + 256 < hi) next = load_jac_rec(src + k + 256);
+        acc = jac_add(f, acc, cur);
+      }
+    }
   }
   store_jac_rec(&sh[threadIdx.x], acc);
   __syncthreads();
