@@ -9,7 +9,7 @@
 // reduced), and the conversion to Jacobian coordinates returns canonical ones.
 #pragma once
 
-#include "zk_curve.hpp"
+#include "zk_curve_wave.hpp"
 
 namespace zk {
 
@@ -80,6 +80,48 @@ __device__ __forceinline__ Xyzz xyzz_madd(const F& f, const Xyzz& p, const Aff& 
   r.y = fsub_lazy(f, mont_raw(f, rr, fsub_for_mul(f, qq, r.x)), mont_raw(f, p.y, ppp));
   r.zz = mont_raw(f, p.zz, pp);
   r.zzz = mont_raw(f, p.zzz, ppp);
+  return r;
+}
+
+// P + Q, add-2007-bl (jac_add in zk_curve.hpp), lazily reduced for the MSM's internal sums:
+// coordinates below 2p with normalized limbs in and out, products without their final
+// subtraction, H and r = 2(S2 - S1) brought below 2p by signed combinations (for their zero tests),
+// the outputs likewise. The same residues as jac_add; P = Q doubles, P = -Q gives the identity.
+template <class F>
+__device__ __forceinline__ Jac jac_add_lazy(const F& f, const Jac& p, const Jac& q) {
+  if (fe_is_zero_lazy(f, p.z)) return q;
+  if (fe_is_zero_lazy(f, q.z)) return p;
+  const Fe z1z1 = sqr_raw(f, p.z), z2z2 = sqr_raw(f, q.z);
+  const Fe u1 = mont_raw(f, p.x, z2z2), u2 = mont_raw(f, q.x, z1z1);
+  const Fe s1 = mont_raw(f, mont_raw(f, p.y, q.z), z2z2);
+  const Fe s2 = mont_raw(f, mont_raw(f, q.y, p.z), z1z1);
+  Acc64 hd = lc_start(f, 2);  // H + 2p in (0, 4p)
+  lc_add(hd, 1, u2);
+  lc_add(hd, -1, u1);
+  const Fe h = lc_finish(f, hd);
+  Acc64 rd = lc_start(f, 4);  // r + 4p in (0, 8p)
+  lc_add(rd, 2, s2);
+  lc_add(rd, -2, s1);
+  const Fe rr = lc_finish(f, rd);
+  if (fe_is_zero_lazy(f, h)) return fe_is_zero_lazy(f, rr) ? jac_dbl(f, jac_canonical(f, p)) : jac_identity(f);
+  const Fe i = sqr_raw(f, fe_dbl_nc(h));  // (2H)^2: 2H below 4p with limbs below 2^30
+  const Fe j = mont_raw(f, h, i), v = mont_raw(f, u1, i);
+  Acc64 xa = lc_start(f, 6);  // r^2 - J - 2V + 6p in (0, 8p)
+  lc_add(xa, 1, sqr_raw(f, rr));
+  lc_add(xa, -1, j);
+  lc_add(xa, -2, v);
+  Jac r;
+  r.x = lc_finish(f, xa);
+  const Fe m1 = mont_raw(f, rr, fsub_for_mul(f, v, r.x)), m2 = mont_raw(f, s1, j);
+  Acc64 ya = lc_start(f, 4);  // r (V - X3) - 2 S1 J + 4p in (0, 6p)
+  lc_add(ya, 1, m1);
+  lc_add(ya, -2, m2);
+  r.y = lc_finish(f, ya);
+  Acc64 za = lc_start(f, 4);  // (Z1 + Z2)^2 - Z1Z1 - Z2Z2 + 4p in (0, 6p), a product operand
+  lc_add(za, 1, sqr_raw(f, fe_add_nc(p.z, q.z)));
+  lc_add(za, -1, z1z1);
+  lc_add(za, -1, z2z2);
+  r.z = mont_raw(f, lc_carry(za), h);
   return r;
 }
 

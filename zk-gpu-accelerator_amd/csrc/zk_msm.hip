@@ -603,7 +603,7 @@ __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, B
     const Jac a = load_jac_rec(ls ? &l->fsum : &l->lsum), b = load_jac_rec(&r->fsum);
     Jac joined;
     if constexpr (kRow) joined = jac_canonical(f, jac_add_wave(f, a, b));
-    else joined = jac_add(f, a, b);
+    else joined = jac_add_lazy(f, a, b);
     if (!lead) return;
     if (ls && rs) {
       store_jac_rec(&l->fsum, joined);
@@ -821,7 +821,7 @@ __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in
   }
   if (add) {
     const JacRec* left = single[1] ? &in[t - 1].fsum : &in[t - 1].lsum;
-    store_bucket(buckets + fk[2], jac_add(f, load_jac_rec(left), load_jac_rec(&me.fsum)));
+    store_bucket(buckets + fk[2], jac_add_lazy(f, load_jac_rec(left), load_jac_rec(&me.fsum)));
   }
   if (r.flags) red[t] = r;
   else red[t].flags = 0;
@@ -865,8 +865,8 @@ __global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ 
   const FqIlp f;
   Jac r = jac_identity(f), t = jac_identity(f);
   for (int i = seg_len - 1; i >= 0; --i) {
-    r = jac_add(f, r, load_bucket(f, b + i));
-    t = jac_add(f, t, r);
+    r = jac_add_lazy(f, r, load_bucket(f, b + i));
+    t = jac_add_lazy(f, t, r);
   }
   store_jac_rec(tot + g, t);
   store_jac_rec(run + g, r);
@@ -912,7 +912,7 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
       for (; k < hi; k += 256) {
         const Jac cur = next;
         if (k + 256 < hi) next = load_jac_rec(src + seg(k + 256));
-        acc = jac_add(f, acc, cur);
+        acc = jac_add_lazy(f, acc, cur);
       }
     }
   } else {
@@ -927,7 +927,7 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
       for (; k < hi; k += 256) {
         const Jac cur = next;
         if (k + 256 < hi) next = load_jac_rec(src + k + 256);
-        acc = jac_add(f, acc, cur);
+        acc = jac_add_lazy(f, acc, cur);
       }
     }
   }
@@ -939,7 +939,7 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
   for (int off = 128; off > 8; off >>= 1) {
     if ((int)threadIdx.x < off) {
       Jac a = load_jac_rec(&sh[threadIdx.x]);
-      a = jac_add(f, a, load_jac_rec(&sh[threadIdx.x + off]));
+      a = jac_add_lazy(f, a, load_jac_rec(&sh[threadIdx.x + off]));
       store_jac_rec(&sh[threadIdx.x], a);
     }
     __syncthreads();
