@@ -583,13 +583,16 @@ struct alignas(16) RunSummary {
 };
 constexpr uint32_t kValid = 1, kSingle = 2;
 
+// Who performs a merge's addition: one thread, the 16 lanes of a row (wave-parallel formulas), or
+// a whole wavefront (limb-sliced formulas); the first lane of a row or wavefront does the writes.
+enum MergeBy { kByThread, kByRow, kByWave };
+
 // Merges the summary at *l with the one at *r (the range right after it), writing every run that
-// becomes bounded on both sides. At most one point addition: by one thread, or (kRow) by the 16
-// lanes of a row with the wave-parallel formulas, whose first lane then does the writes.
-template <bool kRow>
+// becomes bounded on both sides. At most one point addition.
+template <MergeBy kBy>
 __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, BucketRec* __restrict__ buckets) {
   const FqIlp f;
-  const bool lead = !kRow || (threadIdx.x & 15) == 0;
+  const bool lead = kBy == kByThread || (threadIdx.x & (kBy == kByRow ? 15 : 63)) == 0;
   const uint32_t lf = l->flags, rf = r->flags;
   if (!(rf & kValid)) return;
   if (!(lf & kValid)) {
@@ -602,8 +605,14 @@ __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, B
     // the run crossing the boundary
     const Jac a = load_jac_rec(ls ? &l->fsum : &l->lsum), b = load_jac_rec(&r->fsum);
     Jac joined;
-    if constexpr (kRow) joined = jac_canonical(f, jac_add_wave(f, a, b));
-    else joined = jac_add_lazy(f, a, b);
+    if constexpr (kBy == kByRow) {
+      joined = jac_canonical(f, jac_add_wave(f, a, b));
+    } else if constexpr (kBy == kByWave) {
+      const SlLane c = sl_lane<FqConst>();
+      joined = sl_to_jac<FqConst>(c, sl_add<FqConst>(c, sl_from_jac(a), sl_from_jac(b)));
+    } else {
+      joined = jac_add_lazy(f, a, b);
+    }
     if (!lead) return;
     if (ls && rs) {
       store_jac_rec(&l->fsum, joined);
@@ -633,20 +642,24 @@ __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, B
 
 // Tree over the first `width` (a power of two) summaries of a workgroup in LDS; the result is left
 // in sh[0]. Levels with more merges than the workgroup has 16-lane rows run one merge per thread;
-// the upper levels, where a thread-serial addition would leave the level's latency at 16 products
-// in a row, run one merge per row (5 product levels).
+// the upper levels, where a thread-serial addition (9 µs) would set the level's latency, one merge
+// per row (wave-parallel, 6.7 µs), and the last levels, with no more merges than wavefronts, one
+// per wavefront (limb-sliced, 2.7 µs).
 __device__ __forceinline__ void merge_tree(RunSummary* sh, int width, BucketRec* __restrict__ buckets) {
-  const int rows = (int)blockDim.x >> 4;
+  const int rows = (int)blockDim.x >> 4, waves = (int)blockDim.x >> 6;
   for (int d = 1; d < width; d <<= 1) {
     __syncthreads();
     const int merges = width / (2 * d);
     if (merges > rows) {
       // merge i on thread i: the active lanes fill the leading waves, and the rest skip the addition
       const int i = (int)threadIdx.x;
-      if (i < merges) merge_runs<false>(&sh[2 * d * i], &sh[2 * d * i + d], buckets);
-    } else {
+      if (i < merges) merge_runs<kByThread>(&sh[2 * d * i], &sh[2 * d * i + d], buckets);
+    } else if (merges > waves) {
       const int row = (int)threadIdx.x >> 4;
-      if (row < merges) merge_runs<true>(&sh[2 * d * row], &sh[2 * d * row + d], buckets);
+      if (row < merges) merge_runs<kByRow>(&sh[2 * d * row], &sh[2 * d * row + d], buckets);
+    } else {
+      const int wave = (int)threadIdx.x >> 6;
+      if (wave < merges) merge_runs<kByWave>(&sh[2 * d * wave], &sh[2 * d * wave + d], buckets);
     }
   }
   __syncthreads();
