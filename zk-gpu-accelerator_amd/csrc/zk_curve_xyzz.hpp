@@ -125,10 +125,86 @@ __device__ __forceinline__ Jac jac_add_lazy(const F& f, const Jac& p, const Jac&
   return r;
 }
 
+// 2P, dbl-2008-s-1 (a = 0): U = 2Y, V = U^2, W = U V, S = X V, M = 3X^2, X3 = M^2 - 2S,
+// Y3 = M (S - X3) - W Y, ZZ3 = V ZZ, ZZZ3 = W ZZZ. For coordinates below 2p; canonical results.
+// (Only the rare equal-operand case of xyzz_add_lazy takes it.)
+template <class F>
+__device__ __forceinline__ Xyzz xyzz_dbl(const F& f, const Xyzz& p0) {
+  Xyzz p;
+  p.x = canonical(f, p0.x);
+  p.y = canonical(f, p0.y);
+  p.zz = canonical(f, p0.zz);
+  p.zzz = canonical(f, p0.zzz);
+  if (fe_is_zero(p.zz) || fe_is_zero(p.y)) return xyzz_identity(f);
+  const Fe u = fdbl(f, p.y);
+  const Fe v = mont_sqr(f, u);
+  const Fe w = mont(f, u, v);
+  const Fe s = mont(f, p.x, v);
+  const Fe x2 = mont_sqr(f, p.x);
+  const Fe m = fadd(f, fdbl(f, x2), x2);
+  Xyzz r;
+  r.x = fsub(f, mont_sqr(f, m), fdbl(f, s));
+  r.y = fsub(f, mont(f, m, fsub(f, s, r.x)), mont(f, w, p.y));
+  r.zz = mont(f, v, p.zz);
+  r.zzz = mont(f, w, p.zzz);
+  return r;
+}
+
+// P + Q, add-2008-s: U1 = X1 ZZ2, U2 = X2 ZZ1, S1 = Y1 ZZZ2, S2 = Y2 ZZZ1, P = U2 - U1,
+// R = S2 - S1, PP = P^2, PPP = P PP, Q = U1 PP, X3 = R^2 - PPP - 2Q, Y3 = R (Q - X3) - S1 PPP,
+// ZZ3 = ZZ1 ZZ2 PP, ZZZ3 = ZZZ1 ZZZ2 PPP (14 products against add-2007-bl's 16). Lazily reduced
+// like xyzz_madd: coordinates below 2p with normalized limbs in and out. P = Q doubles, P = -Q
+// gives the identity.
+template <class F>
+__device__ __forceinline__ Xyzz xyzz_add_lazy(const F& f, const Xyzz& p, const Xyzz& q) {
+  if (fe_is_zero_lazy(f, p.zz)) return q;
+  if (fe_is_zero_lazy(f, q.zz)) return p;
+  const Fe u1 = mont_raw(f, p.x, q.zz), u2 = mont_raw(f, q.x, p.zz);
+  const Fe s1 = mont_raw(f, p.y, q.zzz), s2 = mont_raw(f, q.y, p.zzz);
+  Acc64 pd = lc_start(f, 2);  // P + 2p in (0, 4p)
+  lc_add(pd, 1, u2);
+  lc_add(pd, -1, u1);
+  const Fe pp0 = lc_finish(f, pd);
+  const Fe rr = fsub_norm(f, s2, s1);  // R + 2p, below 4p
+  if (fe_is_zero_lazy(f, pp0)) {
+    Acc64 rd = lc_start(f, 2);
+    lc_add(rd, 1, s2);
+    lc_add(rd, -1, s1);
+    return fe_is_zero_lazy(f, lc_finish(f, rd)) ? xyzz_dbl(f, p) : xyzz_identity(f);
+  }
+  const Fe pp = sqr_raw(f, pp0);
+  const Fe ppp = mont_raw(f, pp0, pp);
+  const Fe qq = mont_raw(f, u1, pp);
+  Acc64 xa = lc_start(f, 6);  // R^2 - PPP - 2Q + 6p in (0, 8p)
+  lc_add(xa, 1, sqr_raw(f, rr));
+  lc_add(xa, -1, ppp);
+  lc_add(xa, -2, qq);
+  Xyzz r;
+  r.x = lc_finish(f, xa);
+  Acc64 ya = lc_start(f, 2);  // R (Q - X3) - S1 PPP + 2p in (0, 4p)
+  lc_add(ya, 1, mont_raw(f, rr, fsub_for_mul(f, qq, r.x)));
+  lc_add(ya, -1, mont_raw(f, s1, ppp));
+  r.y = lc_finish(f, ya);
+  r.zz = mont_raw(f, mont_raw(f, p.zz, q.zz), pp);
+  r.zzz = mont_raw(f, mont_raw(f, p.zzz, q.zzz), ppp);
+  return r;
+}
+
+// A Jacobian point (coordinates below 2p) in XYZZ coordinates: (X, Y, Z^2, Z^3).
+template <class F>
+__device__ __forceinline__ Xyzz jac_to_xyzz(const F& f, const Jac& p) {
+  Xyzz r;
+  r.x = p.x;
+  r.y = p.y;
+  r.zz = sqr_raw(f, p.z);
+  r.zzz = mont_raw(f, r.zz, p.z);
+  return r;
+}
+
 // The same point in Jacobian coordinates: (X ZZ^2, Y ZZZ^2, ZZZ); the identity maps to (1, 1, 0).
 template <class F>
 __device__ __forceinline__ Jac xyzz_to_jac(const F& f, const Xyzz& p) {
-  if (fe_is_zero(p.zz)) return jac_identity(f);
+  if (fe_is_zero_lazy(f, p.zz)) return jac_identity(f);
   Jac r;
   r.x = mont(f, p.x, mont_sqr(f, p.zz));
   r.y = mont(f, p.y, mont_sqr(f, p.zzz));

@@ -176,6 +176,34 @@ __device__ __forceinline__ void store_bucket(BucketRec* p, const Jac& a) {
   store_bucket_words(p, w);
 }
 
+// The bucket in XYZZ coordinates (Jacobian records converted: 2 products).
+template <class F>
+__device__ __forceinline__ Xyzz load_bucket_xyzz(const F& f, const BucketRec* p) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  uint32_t w[36];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const uint4 v = q[i];
+    w[4 * i] = v.x, w[4 * i + 1] = v.y, w[4 * i + 2] = v.z, w[4 * i + 3] = v.w;
+  }
+  Xyzz a;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    a.x.l[i] = w[i];
+    a.y.l[i] = w[NL + i];
+    a.zz.l[i] = w[2 * NL + i];
+    a.zzz.l[i] = w[3 * NL + i];
+  }
+  if (w[35] & kJacMark) {
+    Jac j;
+    j.x = a.x;
+    j.y = a.y;
+    j.z = a.zz;
+    return jac_to_xyzz(f, j);
+  }
+  return a;
+}
+
 // The bucket as a Jacobian point (XYZZ sums converted: 4 products).
 template <class F>
 __device__ __forceinline__ Jac load_bucket(const F& f, const BucketRec* p) {
@@ -876,13 +904,14 @@ __global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ 
   const int64_t win = local >> job.log_segs, s = local & ((1LL << job.log_segs) - 1);
   const BucketRec* b = buckets + job.key_base + (win << (job.c - 1)) + s * seg_len;
   const FqIlp f;
-  Jac r = jac_identity(f), t = jac_identity(f);
+  // XYZZ running sums (14 products an addition, no conversion per bucket), Jacobian out
+  Xyzz r = xyzz_identity(f), t = xyzz_identity(f);
   for (int i = seg_len - 1; i >= 0; --i) {
-    r = jac_add_lazy(f, r, load_bucket(f, b + i));
-    t = jac_add_lazy(f, t, r);
+    r = xyzz_add_lazy(f, r, load_bucket_xyzz(f, b + i));
+    t = xyzz_add_lazy(f, t, r);
   }
-  store_jac_rec(tot + g, t);
-  store_jac_rec(run + g, r);
+  store_jac_rec(tot + g, xyzz_to_jac(f, t));
+  store_jac_rec(run + g, xyzz_to_jac(f, r));
 }
 
 // Segments each thread adds serially. The rows' additions are a latency chain (the serial sums,
