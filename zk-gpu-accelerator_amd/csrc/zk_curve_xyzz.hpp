@@ -77,7 +77,12 @@ __device__ __forceinline__ Xyzz xyzz_madd(const F& f, const Xyzz& p, const Aff& 
   lc_add(xa, -2, qq);
   Xyzz r;
   r.x = lc_finish(f, xa);
-  r.y = fsub_lazy(f, mont_raw(f, rr, fsub_for_mul(f, qq, r.x)), mont_raw(f, p.y, ppp));
+  if constexpr (UsesIlp<F>::value) {
+    r.y = fsub_lazy(f, mont_raw(f, rr, fsub_for_mul(f, qq, r.x)), mont_raw(f, p.y, ppp));
+  } else {
+    // R (Q - X3) + (2p - Y1) PPP with one reduction
+    r.y = mont_raw_sum2(f, fsub_for_mul(f, qq, r.x), rr, fsub_norm(f, fe_zero(), p.y), ppp);
+  }
   r.zz = mont_raw(f, p.zz, pp);
   r.zzz = mont_raw(f, p.zzz, ppp);
   return r;
