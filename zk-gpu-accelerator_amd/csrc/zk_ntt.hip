@@ -471,8 +471,7 @@ int ntt_run(int field_id, const uint64_t* in, uint64_t* out, int log_n, const ui
   std::vector<NttPassDesc> ds = pass_descs(in, out, tmp, log_n, tw, pre, post, post_table, q);
   return with_field(field_id, [&](auto f) {
     for (const NttPassDesc& d : ds) {
-      Prof prof("ntt_pass", stream);
-      hipLaunchKernelGGL((k_ntt_pass<decltype(f)>), dim3((unsigned)pass_tiles(d)), dim3(kThreads), 0, stream, f, d);
+      launch_timed("ntt_pass", k_ntt_pass<decltype(f)>, dim3((unsigned)pass_tiles(d)), dim3(kThreads), 0, stream, f, d);
       ZK_LAUNCH_CHECK("ntt pass");
     }
     return 0;
@@ -562,9 +561,8 @@ int batch_plan_create(const std::vector<TransformSpec>& specs, hipStream_t strea
 int batch_plan_launch(const BatchPlan* plan, hipStream_t stream) {
   const FrConst f;
   for (size_t i = 0; i < plan->counts.size(); ++i) {
-    Prof prof("ntt_pass_multi", stream);
-    hipLaunchKernelGGL((k_ntt_pass_multi<FrConst>), dim3((unsigned)plan->tiles[i]), dim3(kThreads), 0, stream, f,
-                       plan->d_desc[i], plan->counts[i]);
+    launch_timed("ntt_pass_multi", k_ntt_pass_multi<FrConst>, dim3((unsigned)plan->tiles[i]), dim3(kThreads), 0, stream, f,
+                 static_cast<const NttPassDesc*>(plan->d_desc[i]), plan->counts[i]);
     ZK_LAUNCH_CHECK("batched ntt pass");
   }
   return 0;

@@ -125,20 +125,24 @@ static std::mutex g_prof_mu;
 static bool g_prof_on = false;
 static std::vector<ProfRecord> g_prof;
 
-Prof::Prof(const char* name, hipStream_t stream) : name_(name), stream_(stream) {
-  std::lock_guard<std::mutex> lk(g_prof_mu);
-  if (!g_prof_on) return;
+bool prof_begin(hipStream_t stream, hipEvent_t* start, hipEvent_t* stop) {
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (!g_prof_on) return false;
+  }
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(stream_, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
-  if (hipEventCreate(&start_) != hipSuccess || hipEventRecord(start_, stream_) != hipSuccess) start_ = nullptr;
+  if (hipStreamIsCapturing(stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return false;
+  if (hipEventCreate(start) != hipSuccess) return false;
+  if (hipEventCreate(stop) != hipSuccess) {
+    (void)hipEventDestroy(*start);
+    return false;
+  }
+  return true;
 }
 
-Prof::~Prof() {
-  if (!start_) return;
-  hipEvent_t end = nullptr;
-  if (hipEventCreate(&end) != hipSuccess || hipEventRecord(end, stream_) != hipSuccess) return;
+void prof_commit(const char* name, hipEvent_t start, hipEvent_t stop) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
-  g_prof.push_back(ProfRecord{name_, start_, end});
+  g_prof.push_back(ProfRecord{name, start, stop});
 }
 
 static int g_cus = 0;

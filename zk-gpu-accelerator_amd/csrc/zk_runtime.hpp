@@ -2,6 +2,7 @@
 // error reporting, the caching device allocator and scratch buffers.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -41,20 +42,24 @@ class Scratch {
 
 int device_cus();  // compute units of the current device
 
-// Opt-in kernel timing (zk_prof_enable): a Prof brackets one launch with HIP
-// events on its stream; zk_prof_read sums the durations per name.
-class Prof {
- public:
-  Prof(const char* name, hipStream_t stream);
-  ~Prof();
-  Prof(const Prof&) = delete;
-  Prof& operator=(const Prof&) = delete;
+// Opt-in kernel timing (zk_prof_enable): launch_timed passes a start and a stop event to
+// hipExtLaunchKernelGGL, so the dispatch itself takes the timestamps (marker packets between
+// kernels would add a gap of about 10 us each); zk_prof_read sums the durations per name.
+// prof_begin creates the two events when timing is on and the stream is not being captured.
+bool prof_begin(hipStream_t stream, hipEvent_t* start, hipEvent_t* stop);
+void prof_commit(const char* name, hipEvent_t start, hipEvent_t stop);
 
- private:
-  const char* name_;
-  hipStream_t stream_;
-  hipEvent_t start_ = nullptr;
-};
+template <class Kernel, class... Args>
+void launch_timed(const char* name, Kernel kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t stream,
+                  Args... args) {
+  hipEvent_t start = nullptr, stop = nullptr;
+  if (prof_begin(stream, &start, &stop)) {
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, start, stop, 0, args...);
+    prof_commit(name, start, stop);
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
+  }
+}
 
 }  // namespace zk
 
