@@ -52,7 +52,6 @@
 // The result is a unique group element, so it equals the reference's
 // msm_pippenger for every input; intermediate coordinates never leave the device.
 #include <algorithm>
-#include <mutex>
 #include <vector>
 
 #include "../../include/zkgpu.h"
@@ -1362,14 +1361,11 @@ static int msm_plan_clear(MsmPlan* plan, hipStream_t stream) {
   return 0;
 }
 
-// Enqueues the whole batch; allocates nothing. With `ready`, the caller has cleared the buckets and
-// the points become ready at that event (work on another stream): the launch waits for it just
-// before the accumulation, so the digits and the sort overlap that work.
-int msm_plan_launch(MsmPlan* plan, hipStream_t stream, hipEvent_t ready = nullptr) {
+// Enqueues the whole batch; allocates nothing.
+int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
   if (plan->njobs == 0) return 0;
   const int nj = plan->njobs;
-  if (!ready)
-    if (int rc = msm_plan_clear(plan, stream)) return rc;
+  if (int rc = msm_plan_clear(plan, stream)) return rc;
   {
     hipLaunchKernelGGL(k_signed_digits, dim3((unsigned)plan->blocks), dim3(256), 0, stream, plan->d_jobs, nj, plan->digits);
     ZK_LAUNCH_CHECK("msm digits");
@@ -1403,7 +1399,6 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream, hipEvent_t ready = nullpt
   }
   const int levels = (int)plan->level_max.size();
   if (plan->level_max[0] > plan->max_slots) return fail("internal: MSM run-summary buffer smaller than a level");
-  if (ready) ZK_HIP(hipStreamWaitEvent(stream, ready, 0));
   launch_timed("msm_accumulate", acc_kernel(plan->acc_variant), dim3((unsigned)plan->acc_blocks), dim3(kAccThreads), 0,
                stream, plan->sorted, plan->m_dev, plan->points, plan->buckets, plan->runs[0]);
   ZK_LAUNCH_CHECK("msm accumulate");
@@ -1458,31 +1453,6 @@ int convert_points(const uint64_t* points, int64_t n, AffRec* out, bool with_phi
   return 0;
 }
 
-// A second stream per device for work that overlaps the caller's stream inside one call, with the
-// events of its fork and join; created on first use, kept until the process ends (default
-// priority, non-blocking). One caller at a time uses them (g_side_mu).
-struct SideStream {
-  hipStream_t s = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-};
-static std::mutex g_side_mu;
-constexpr int kMaxDevices = 64;
-static SideStream g_side[kMaxDevices];
-
-static int side_stream(SideStream** out) {
-  int dev = 0;
-  ZK_HIP(hipGetDevice(&dev));
-  if (dev < 0 || dev >= kMaxDevices) return fail("device index beyond the side-stream table");
-  SideStream& ss = g_side[dev];
-  if (!ss.s) {
-    ZK_HIP(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
-    ZK_HIP(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
-    ZK_HIP(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
-  }
-  *out = &ss;
-  return 0;
-}
-
 int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, int k0, uint64_t* out_jac,
            hipStream_t stream) {
   if (n < 0 || n > 0x3fffffff) return fail("MSM size must be in [0, 2^30)");
@@ -1491,6 +1461,7 @@ int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, in
   Scratch sc(stream);
   AffRec* pts = sc.get<AffRec>((size_t)std::max<int64_t>(glv ? 2 * n : n, 1));
   if (!pts) return 1;
+  if (int rc = convert_points(points, n, pts, glv, stream)) return rc;
   MsmJob job{};
   job.scalars = scalars;
   job.n = n;
@@ -1499,27 +1470,7 @@ int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, in
   job.phi_base = n;
   MsmPlan* plan = nullptr;
   if (int rc = msm_plan_create(pts, {job}, k0, stream, &plan)) return rc;
-  int rc = 0;
-  if (plan->njobs == 0) {
-    rc = convert_points(points, n, pts, glv, stream);
-    if (!rc) rc = msm_plan_launch(plan, stream);
-  } else {
-    // The point conversion and the bucket clearing depend on nothing the digits and the sort
-    // produce: they run on a second stream, forked from this one here and joined before the
-    // accumulation (and, whatever fails, before this call returns its memory).
-    std::lock_guard<std::mutex> lock(g_side_mu);
-    SideStream* side = nullptr;
-    rc = side_stream(&side);
-    bool forked = false;
-    if (!rc && hipEventRecord(side->fork, stream) != hipSuccess) rc = fail("MSM fork: event record failed");
-    if (!rc && hipStreamWaitEvent(side->s, side->fork, 0) != hipSuccess) rc = fail("MSM fork: stream wait failed");
-    if (!rc) forked = true;
-    if (!rc) rc = msm_plan_clear(plan, side->s);  // (beside the digits: a write stream against a compute kernel)
-    if (!rc) rc = convert_points(points, n, pts, glv, side->s);
-    if (forked && hipEventRecord(side->join, side->s) != hipSuccess && !rc) rc = fail("MSM join: event record failed");
-    if (!rc) rc = msm_plan_launch(plan, stream, side->join);
-    if (rc && forked) (void)hipStreamWaitEvent(stream, side->join, 0);
-  }
+  int rc = msm_plan_launch(plan, stream);
   msm_plan_destroy(plan);  // blocks return to this stream's cache: later work on it is ordered after the launch
   return rc;
 }
