@@ -544,7 +544,19 @@ __device__ __forceinline__ Fe form256_to_plain(const F& f, const Fe& x) {
 
 template <class F>
 __device__ __forceinline__ Fe form256_to_261(const F& f, const Fe& x) {
-  return mont(f, x, fe_const<F>(f.c266));
+  if constexpr (HasP4r<F>::value) {
+    // x 2^5 mod p without a product (built-in moduli, x < 2^256 with normalized limbs): the limbs
+    // shifted up 5 bits, then less a quotient estimated from the top limb (lc_finish: exact for
+    // values below 2^261, the result below 2p), then one conditional subtraction.
+    Acc64 a;
+    a.l[0] = (int64_t)((x.l[0] << 5) & LMASK);
+#pragma unroll
+    for (int i = 1; i < NL - 1; ++i) a.l[i] = (int64_t)(((x.l[i] << 5) | (x.l[i - 1] >> (LB - 5))) & LMASK);
+    a.l[NL - 1] = (int64_t)((x.l[NL - 1] << 5) | (x.l[NL - 2] >> (LB - 5)));
+    return canonical(f, lc_finish(f, a));
+  } else {
+    return mont(f, x, fe_const<F>(f.c266));
+  }
 }
 
 template <class F>
