@@ -79,6 +79,44 @@ struct alignas(16) JacRec {
   uint32_t w[28];  // x 0..8, y 9..17, z 18..26 (29-bit limbs)
 };
 
+// An XYZZ point (x, y, zz, zzz: nine 29-bit limbs each), 144 bytes.
+struct alignas(16) XyzzRec {
+  uint32_t w[36];
+};
+
+__device__ __forceinline__ Xyzz load_xyzz_rec(const XyzzRec* p) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  uint32_t w[36];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const uint4 v = q[i];
+    w[4 * i] = v.x, w[4 * i + 1] = v.y, w[4 * i + 2] = v.z, w[4 * i + 3] = v.w;
+  }
+  Xyzz a;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    a.x.l[i] = w[i];
+    a.y.l[i] = w[NL + i];
+    a.zz.l[i] = w[2 * NL + i];
+    a.zzz.l[i] = w[3 * NL + i];
+  }
+  return a;
+}
+
+__device__ __forceinline__ void store_xyzz_rec(XyzzRec* p, const Xyzz& a) {
+  uint4* q = reinterpret_cast<uint4*>(p);
+  uint32_t w[36];
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    w[i] = a.x.l[i];
+    w[NL + i] = a.y.l[i];
+    w[2 * NL + i] = a.zz.l[i];
+    w[3 * NL + i] = a.zzz.l[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
 // The raw words of an AffRec, loaded ahead of their use.
 struct AffRaw {
   uint4 a, b, c, d;
@@ -894,8 +932,8 @@ __global__ void __launch_bounds__(kCombineThreads) k_combine(const RunSummary* _
 // Leaf of the bucket reduction: segment s of a window covers buckets [sL, sL + L)
 // (bucket b holds the digit value b + 1): T = sum (i + 1) B_{sL+i}, R = sum B_{sL+i}.
 __global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ jobs, int njobs, int64_t segments,
-                                                     const BucketRec* __restrict__ buckets, JacRec* __restrict__ tot,
-                                                     JacRec* __restrict__ run) {
+                                                     const BucketRec* __restrict__ buckets, XyzzRec* __restrict__ tot,
+                                                     XyzzRec* __restrict__ run) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= segments) return;
   const MsmJob& job = jobs[find_job<&MsmJob::seg_base>(jobs, njobs, g)];
@@ -904,14 +942,14 @@ __global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ 
   const int64_t win = local >> job.log_segs, s = local & ((1LL << job.log_segs) - 1);
   const BucketRec* b = buckets + job.key_base + (win << (job.c - 1)) + s * seg_len;
   const FqIlp f;
-  // XYZZ running sums (14 products an addition, no conversion per bucket), Jacobian out
+  // XYZZ running sums (14 products an addition, no conversion per bucket)
   Xyzz r = xyzz_identity(f), t = xyzz_identity(f);
   for (int i = seg_len - 1; i >= 0; --i) {
     r = xyzz_add_lazy(f, r, load_bucket_xyzz(f, b + i));
     t = xyzz_add_lazy(f, t, r);
   }
-  store_jac_rec(tot + g, xyzz_to_jac(f, t));
-  store_jac_rec(run + g, xyzz_to_jac(f, r));
+  store_xyzz_rec(tot + g, t);
+  store_xyzz_rec(run + g, r);
 }
 
 // Segments each thread adds serially. The rows' additions are a latency chain (the serial sums,
@@ -928,9 +966,10 @@ __device__ __forceinline__ int red_rows(const MsmJob& job) { return job.log_segs
 
 // One workgroup per (job, window, row, chunk): 256 threads x kRedPerThread selected segments, then a tree.
 __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ jobs, int njobs,
-                                                     const JacRec* __restrict__ tot, const JacRec* __restrict__ run,
+                                                     const XyzzRec* __restrict__ tot, const XyzzRec* __restrict__ run,
                                                      JacRec* __restrict__ parts) {
-  __shared__ JacRec sh[256];
+  __shared__ XyzzRec sh[256];
+  __shared__ JacRec shj[16];
   const MsmJob& job = jobs[find_job<&MsmJob::part_base>(jobs, njobs, blockIdx.x)];
   const int64_t local = (int64_t)blockIdx.x - job.part_base;
   const int per = red_rows(job);
@@ -940,63 +979,65 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
   const int j = (int)(row % per);
   const int64_t segs = 1LL << job.log_segs;
   const FqIlp f;
-  Jac acc = jac_identity(f);
+  Xyzz acc = xyzz_identity(f);
   const int64_t lo = chunk * kRedChunk;
   if (j < job.log_segs) {
     // the k-th segment with bit j set, k in this chunk: whole runs of 2^j segments, so that
     // neighbouring lanes never skip additions the wavefront executes anyway
-    const JacRec* src = run + job.seg_base + win * segs;
+    const XyzzRec* src = run + job.seg_base + win * segs;
     const int64_t hi = min(segs >> 1, lo + (int64_t)kRedChunk), low = (1LL << j) - 1;
     auto seg = [&](int64_t k) { return ((k >> j) << (j + 1)) + (1LL << j) + (k & low); };
     int64_t k = lo + threadIdx.x;
     if (k < hi) {
-      Jac next = load_jac_rec(src + seg(k));  // the next segment's load overlaps this addition
+      Xyzz next = load_xyzz_rec(src + seg(k));  // the next segment's load overlaps this addition
       for (; k < hi; k += 256) {
-        const Jac cur = next;
-        if (k + 256 < hi) next = load_jac_rec(src + seg(k + 256));
-        acc = jac_add_lazy(f, acc, cur);
+        const Xyzz cur = next;
+        if (k + 256 < hi) next = load_xyzz_rec(src + seg(k + 256));
+        acc = xyzz_add_lazy(f, acc, cur);
       }
     }
   } else {
     // the T sum over one half of the segments
     const int64_t first = (segs + 1) >> 1;
     const int64_t base = j == job.log_segs ? 0 : first, count = j == job.log_segs ? first : segs - first;
-    const JacRec* src = tot + job.seg_base + win * segs + base;
+    const XyzzRec* src = tot + job.seg_base + win * segs + base;
     const int64_t hi = min(count, lo + (int64_t)kRedChunk);
     int64_t k = lo + threadIdx.x;
     if (k < hi) {
-      Jac next = load_jac_rec(src + k);
+      Xyzz next = load_xyzz_rec(src + k);
       for (; k < hi; k += 256) {
-        const Jac cur = next;
-        if (k + 256 < hi) next = load_jac_rec(src + k + 256);
-        acc = jac_add_lazy(f, acc, cur);
+        const Xyzz cur = next;
+        if (k + 256 < hi) next = load_xyzz_rec(src + k + 256);
+        acc = xyzz_add_lazy(f, acc, cur);
       }
     }
   }
-  store_jac_rec(&sh[threadIdx.x], acc);
+  store_xyzz_rec(&sh[threadIdx.x], acc);
   __syncthreads();
-  // a tree over the 256 partial sums: one addition per thread down to 16 sums, then the last
-  // levels on the four wavefronts with the limb-sliced formulas (an addition in about a third of
-  // one lane's time)
+  // a tree over the 256 partial sums: one XYZZ addition per thread down to 16 sums, which become
+  // Jacobian points for the last levels on the four wavefronts with the limb-sliced formulas (an
+  // addition in about a third of one lane's time)
   for (int off = 128; off > 8; off >>= 1) {
     if ((int)threadIdx.x < off) {
-      Jac a = load_jac_rec(&sh[threadIdx.x]);
-      a = jac_add_lazy(f, a, load_jac_rec(&sh[threadIdx.x + off]));
-      store_jac_rec(&sh[threadIdx.x], a);
+      Xyzz a = load_xyzz_rec(&sh[threadIdx.x]);
+      a = xyzz_add_lazy(f, a, load_xyzz_rec(&sh[threadIdx.x + off]));
+      store_xyzz_rec(&sh[threadIdx.x], a);
     }
     __syncthreads();
   }
+  if (threadIdx.x < 16) store_jac_rec(&shj[threadIdx.x], xyzz_to_jac(f, load_xyzz_rec(&sh[threadIdx.x])));
+  __syncthreads();
   const SlLane c = sl_lane<FqConst>();
   const int wave = (int)threadIdx.x >> 6;
   for (int off = 8; off > 0; off >>= 1) {
     for (int i = wave; i < off; i += 4) {
-      const SlJac a = sl_add<FqConst>(c, sl_from_jac(load_jac_rec(&sh[i])), sl_from_jac(load_jac_rec(&sh[i + off])));
+      const SlJac a = sl_add<FqConst>(c, sl_from_jac(load_jac_rec(&shj[i])), sl_from_jac(load_jac_rec(&shj[i + off])));
       const Jac r = sl_to_jac<FqConst>(c, a);
-      if ((threadIdx.x & 63) == 0) store_jac_rec(&sh[i], off == 1 ? jac_canonical(f, r) : r);
+      if ((threadIdx.x & 63) == 0) store_jac_rec(&shj[i], off == 1 ? jac_canonical(f, r) : r);
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) parts[blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) parts[blockIdx.x] = shj[0];
 }
 
 // Window sums U = sum T_s + sum_j 2^(j + log_seg) Q_j, in two launches. k_window_terms: four
@@ -1167,7 +1208,8 @@ struct MsmPlan {
   uint2 *staged = nullptr, *sorted = nullptr;
   BucketRec* buckets = nullptr;
   RunSummary* runs[2] = {nullptr, nullptr};
-  JacRec *tot = nullptr, *run = nullptr, *parts_buf = nullptr, *wparts = nullptr, *wsum = nullptr;
+  XyzzRec *tot = nullptr, *run = nullptr;
+  JacRec *parts_buf = nullptr, *wparts = nullptr, *wsum = nullptr;
 };
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1288,8 +1330,8 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   L->buckets = take(sizeof(BucketRec) * (size_t)keys);
   L->r0 = take(sizeof(RunSummary) * (size_t)max_slots);
   L->r1 = take(sizeof(RunSummary) * (size_t)max_slots1);
-  L->tot = take(sizeof(JacRec) * (size_t)segs);
-  L->run = take(sizeof(JacRec) * (size_t)segs);
+  L->tot = take(sizeof(XyzzRec) * (size_t)segs);
+  L->run = take(sizeof(XyzzRec) * (size_t)segs);
   L->parts = take(sizeof(JacRec) * (size_t)parts);
   L->wparts = take(sizeof(JacRec) * (size_t)wins * kWinGroups);
   L->wsum = take(sizeof(JacRec) * (size_t)wins);
@@ -1328,8 +1370,8 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   plan->buckets = reinterpret_cast<BucketRec*>(base + L.buckets);
   plan->runs[0] = reinterpret_cast<RunSummary*>(base + L.r0);
   plan->runs[1] = reinterpret_cast<RunSummary*>(base + L.r1);
-  plan->tot = reinterpret_cast<JacRec*>(base + L.tot);
-  plan->run = reinterpret_cast<JacRec*>(base + L.run);
+  plan->tot = reinterpret_cast<XyzzRec*>(base + L.tot);
+  plan->run = reinterpret_cast<XyzzRec*>(base + L.run);
   plan->parts_buf = reinterpret_cast<JacRec*>(base + L.parts);
   plan->wparts = reinterpret_cast<JacRec*>(base + L.wparts);
   plan->wsum = reinterpret_cast<JacRec*>(base + L.wsum);
