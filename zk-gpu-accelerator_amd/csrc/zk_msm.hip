@@ -474,6 +474,47 @@ __global__ void __launch_bounds__(256) k_part_chunks(const uint32_t* __restrict_
   if (p < nparts) chunks[p] = (totals[p] + kChunkB - 1) / kChunkB;
 }
 
+// The sort's offsets in one workgroup when the partitions are few (a single MSM has a few hundred):
+// starts = the exclusive scan of the partition totals and *m their sum, chunk_base = the exclusive
+// scan of the partitions' pass-B chunk counts and chunk_base[nparts] their sum. The same as two
+// device-wide scans and k_part_chunks, in one launch instead of five.
+constexpr int kOffThreads = 1024;
+constexpr int64_t kOffMaxParts = 16 * kOffThreads;
+
+__global__ void __launch_bounds__(kOffThreads) k_part_offsets(const uint32_t* __restrict__ totals, int64_t nparts,
+                                                              uint32_t* __restrict__ starts, uint32_t* __restrict__ m,
+                                                              uint32_t* __restrict__ chunk_base) {
+  __shared__ uint32_t sa[kOffThreads], sb[kOffThreads];
+  uint32_t carry_a = 0, carry_b = 0;  // the same in every thread
+  for (int64_t base = 0; base < nparts; base += kOffThreads) {
+    const int64_t i = base + threadIdx.x;
+    const uint32_t v = i < nparts ? totals[i] : 0u;
+    const uint32_t c = (v + kChunkB - 1) / kChunkB;
+    sa[threadIdx.x] = v;
+    sb[threadIdx.x] = c;
+    __syncthreads();
+    for (int d = 1; d < kOffThreads; d <<= 1) {  // inclusive scans
+      const uint32_t xa = (int)threadIdx.x >= d ? sa[threadIdx.x - d] : 0u;
+      const uint32_t xb = (int)threadIdx.x >= d ? sb[threadIdx.x - d] : 0u;
+      __syncthreads();
+      sa[threadIdx.x] += xa;
+      sb[threadIdx.x] += xb;
+      __syncthreads();
+    }
+    if (i < nparts) {
+      starts[i] = carry_a + sa[threadIdx.x] - v;
+      chunk_base[i] = carry_b + sb[threadIdx.x] - c;
+    }
+    carry_a += sa[kOffThreads - 1];
+    carry_b += sb[kOffThreads - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *m = carry_a;
+    chunk_base[nparts] = carry_b;
+  }
+}
+
 // The (partition, chunk) of pass-B workgroup g, or false past the last chunk.
 struct PartChunk {
   int64_t p, lo, hi, bkey0;
@@ -1417,17 +1458,26 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
     hipLaunchKernelGGL(k_sort_columns, dim3(grid_for(plan->nparts * 64, 256)), dim3(256), 0, stream, plan->d_jobs, nj,
                        plan->nparts, plan->counts, plan->totals);
     ZK_LAUNCH_CHECK("msm sort columns");
-    if (int rc = scan_u32_ws(plan->totals, plan->starts, plan->nparts, plan->m_dev, plan->scan_ws, stream)) return rc;
+    const bool few = plan->nparts <= kOffMaxParts;
+    if (few) {
+      hipLaunchKernelGGL(k_part_offsets, dim3(1), dim3(kOffThreads), 0, stream, plan->totals, plan->nparts, plan->starts,
+                         plan->m_dev, plan->chunk_base);
+      ZK_LAUNCH_CHECK("msm sort offsets");
+    } else if (int rc = scan_u32_ws(plan->totals, plan->starts, plan->nparts, plan->m_dev, plan->scan_ws, stream)) {
+      return rc;
+    }
     hipLaunchKernelGGL(k_sort_place, dim3((unsigned)plan->sort_blocks), dim3(256), 0, stream, plan->d_jobs, nj,
                        plan->digits, plan->counts, plan->starts, plan->staged);
     ZK_LAUNCH_CHECK("msm sort place");
     // pass B: every partition sorted by bucket, chunk by chunk
-    hipLaunchKernelGGL(k_part_chunks, dim3(grid_for(plan->nparts, 256)), dim3(256), 0, stream, plan->totals, plan->nparts,
-                       plan->chunks);
-    ZK_LAUNCH_CHECK("msm sort chunks");
-    if (int rc = scan_u32_ws(plan->chunks, plan->chunk_base, plan->nparts, plan->chunk_base + plan->nparts, plan->scan_ws,
-                             stream))
-      return rc;
+    if (!few) {
+      hipLaunchKernelGGL(k_part_chunks, dim3(grid_for(plan->nparts, 256)), dim3(256), 0, stream, plan->totals, plan->nparts,
+                         plan->chunks);
+      ZK_LAUNCH_CHECK("msm sort chunks");
+      if (int rc = scan_u32_ws(plan->chunks, plan->chunk_base, plan->nparts, plan->chunk_base + plan->nparts, plan->scan_ws,
+                               stream))
+        return rc;
+    }
     hipLaunchKernelGGL(k_part_count, dim3((unsigned)plan->max_chunks), dim3(256), 0, stream, plan->d_jobs, nj, plan->chunk_base,
                        plan->nparts, plan->starts, plan->totals, plan->staged, plan->counts_b, plan->counts_raw);
     ZK_LAUNCH_CHECK("msm sort partition count");
