@@ -137,7 +137,6 @@ def main() -> int:
     gb.prof_read("msm_accumulate", reset=True)
     ms = timed(msm_step, args.steps, 0, gb)
     acc_ms, acc_launches = gb.prof_read("msm_accumulate")
-    fin_ms, fin_launches = gb.prof_read("msm_final")
     gb.prof_read("", reset=True)
     from zkgpu.msm_gpu import _result
 
@@ -154,7 +153,7 @@ def main() -> int:
     peak_tmads = _valu_peak(gb)
     msm_mpts = n / (ms * 1e-3) / 1e6
     log(f"msm 2^{args.log_msm}: {ms:.3f} ms/step, {msm_mpts:.2f} Mpts/s; accumulate {acc_avg:.3f} ms "
-        f"({achieved_tmads:.2f} Tmad/s of {peak_tmads:.2f}); final {fin_ms / max(fin_launches, 1):.3f} ms")
+        f"({achieved_tmads:.2f} Tmad/s of {peak_tmads:.2f})")
 
     traffic = _traffic("k_accumulate", n)
     extras: dict = {}

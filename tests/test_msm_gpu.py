@@ -249,3 +249,23 @@ def test_one_bucket_spans_every_range(chunk):
         msm_gpu.msm_g1_jacobian(pts, gpu_backend.asarray(ints_to_limbs(scalars)), out, chunk_len=chunk)
         got = msm_gpu._result(out)
         assert (got.is_identity() and e == 0) or got.affine() == (G * e).affine(), chunk
+
+
+@pytest.mark.parametrize("chunk", [0, 2])
+def test_run_sums_that_cancel_across_ranges(chunk):
+    """A basis whose second half negates its first (P_{i+h} = -P_i) under equal scalars: each
+    window's bucket sums to the identity, so the last merges of the run-summary trees add a partial
+    sum X to -X, and the identity they produce must read as the identity to every later stage."""
+    n = 1 << 14
+    rng = random.Random(47)
+    half = [rng.randrange(1, R) for _ in range(n // 2)]
+    keys = half + [R - k for k in half]
+    pts = workloads.srs_points(ints_to_limbs(keys))
+    s = rng.randrange(1, R)
+    cases = [[1] * n, [s] * n, [s] * (n - 1) + [rng.randrange(1, R)]]
+    for scalars in cases:
+        e = workloads.closed_form_scalar(keys, scalars)
+        out = gpu_backend.empty((3, 4))
+        msm_gpu.msm_g1_jacobian(pts, gpu_backend.asarray(ints_to_limbs(scalars)), out, chunk_len=chunk)
+        got = msm_gpu._result(out)
+        assert (got.is_identity() and e == 0) or got.affine() == (G * e).affine(), (chunk, scalars[-1])

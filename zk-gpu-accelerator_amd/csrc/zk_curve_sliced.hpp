@@ -187,6 +187,8 @@ __device__ __forceinline__ SlJac sl_from_jac(const Jac& p) {
   return r;
 }
 
+// Coordinates below 2p, not canonical: a zero coordinate comes out as p (the identity as Z = p), so
+// a result handed to code that tests Z == 0 exactly goes through jac_canonical first.
 template <class F>
 __device__ __forceinline__ Jac sl_to_jac(const SlLane& c, const SlJac& s) {
   Jac r;

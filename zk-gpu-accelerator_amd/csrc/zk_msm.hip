@@ -715,8 +715,10 @@ __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, B
     if constexpr (kBy == kByRow) {
       joined = jac_canonical(f, jac_add_wave(f, a, b));
     } else if constexpr (kBy == kByWave) {
+      // canonical: the sliced identity leaves with Z = p, and the thread-serial formulas and the
+      // bucket loads recognise the identity by Z == 0 exactly
       const SlLane c = sl_lane<FqConst>();
-      joined = sl_to_jac<FqConst>(c, sl_add<FqConst>(c, sl_from_jac(a), sl_from_jac(b)));
+      joined = jac_canonical(f, sl_to_jac<FqConst>(c, sl_add<FqConst>(c, sl_from_jac(a), sl_from_jac(b))));
     } else {
       joined = jac_add_lazy(f, a, b);
     }
@@ -1529,8 +1531,7 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
                        plan->windows_total, plan->wparts, plan->wsum);
     ZK_LAUNCH_CHECK("msm window sums");
   }
-  launch_timed("msm_final", k_final, dim3((unsigned)nj), dim3(64), 0, stream, static_cast<const MsmJob*>(plan->d_jobs), nj,
-               static_cast<const JacRec*>(plan->wsum));
+  hipLaunchKernelGGL(k_final, dim3((unsigned)nj), dim3(64), 0, stream, plan->d_jobs, nj, plan->wsum);
   ZK_LAUNCH_CHECK("msm final");
   return 0;
 }
