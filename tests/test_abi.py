@@ -70,6 +70,13 @@ def test_constants(struct, m):
     for name, e in (("c256", 256), ("c266", 266), ("c517", 517), ("c522", 522), ("one", 261)):
         assert c[name] == limbs29(pow(2, e, m)), name
     assert c["p32"] == [(m >> (32 * i)) & 0xFFFFFFFF for i in range(8)]
+    # redundant 4p and 8p: the value, and limbs 0..7 large enough that a + kp - b keeps every
+    # limb non-negative for a normalized b (4p) or a carry-free sum of two normalized values (8p)
+    for name, k, floor in (("p4r", 4, 2**29 - 1), ("p8r", 8, 2**30 - 2)):
+        r = c[name]
+        assert sum(v << (29 * i) for i, v in enumerate(r)) == k * m, name
+        assert all(v >= floor for v in r[:8]) and r[8] >= 0, name
+        assert all(v < 2**32 for v in r), name
 
 
 def mont29(a, b, m):
@@ -96,6 +103,71 @@ def mont29(a, b, m):
         acc >>= 29
     out[8] = acc
     return sum(v << (29 * i) for i, v in enumerate(out)), top
+
+
+def mont29_limbs(A, B, m):
+    """mont29 on raw limb vectors (limbs beyond 29 bits, as the carry-free operands have)."""
+    p = limbs29(m)
+    ninv = (-pow(m, -1, 2**29)) % 2**29
+    mm, acc, top, out = [0] * 9, 0, 0, [0] * 9
+    for k in range(9):
+        acc += sum(A[i] * B[k - i] for i in range(k + 1)) + sum(mm[i] * p[k - i] for i in range(k))
+        mm[k] = (acc & 0xFFFFFFFF) * ninv & ((1 << 29) - 1)
+        acc += mm[k] * p[0]
+        top = max(top, acc)
+        acc >>= 29
+    for k in range(9, 17):
+        acc += sum(A[i] * B[k - i] + mm[i] * p[k - i] for i in range(k - 8, 9))
+        top = max(top, acc)
+        out[k - 9] = acc & ((1 << 29) - 1)
+        acc >>= 29
+    out[8] = acc
+    return sum(v << (29 * i) for i, v in enumerate(out)), top
+
+
+def lc_finish29(L, m):
+    """zk_field.hpp's lc_finish on signed limbs: the quotient from the top limb, then carries."""
+    p = limbs29(m)
+    t = L[8] - 4096
+    q = t // (p[8] + 1) if t > 0 else 0
+    a = [x - q * y for x, y in zip(L, p)]
+    for i in range(8):
+        a[i + 1] += a[i] >> 29
+        a[i] &= (1 << 29) - 1
+    return a
+
+
+@pytest.mark.parametrize("struct,m", [("FrConst", R), ("FqConst", P)])
+def test_wide_butterfly_sums(struct, m):
+    """The NTT's carry-free sums: x0 + x2 and x1 + x3 (each below 4p), their sum reduced by the
+    top-limb quotient, their difference + 4p likewise, and their difference + 8p (p8r) as a
+    product operand, at the largest limbs and values the stage pair can hand over."""
+    c = parse_consts()[struct]
+    rng = random.Random(m % 997)
+    hi = 2 * m - 1  # the largest value below 2p, and the largest normalized limbs
+    vals = [0, 1, m - 1, m, hi, (1 << 253) - 1] + [rng.randrange(2 * m) for _ in range(200)]
+    maxlimbs = [2**29 - 1] * 8 + [limbs29(hi)[8]]
+    tw = [limbs29(m - 1), limbs29(hi), [2**29 - 1] * 8 + [limbs29(hi)[8]]]
+    for _ in range(300):
+        xs = [limbs29(rng.choice(vals)) for _ in range(4)]
+        if rng.random() < 0.2:
+            xs[rng.randrange(4)] = maxlimbs  # limbs at their bound (a value up to ~2^254)
+        y0 = [a + b for a, b in zip(xs[0], xs[2])]
+        y1 = [a + b for a, b in zip(xs[1], xs[3])]
+        v0, v1 = (sum(v << (29 * i) for i, v in enumerate(y)) for y in (y0, y1))
+        s = lc_finish29([a + b for a, b in zip(y0, y1)], m)
+        sv = sum(v << (29 * i) for i, v in enumerate(s))
+        assert sv < 2 * m and sv % m == (v0 + v1) % m and all(0 <= x < 2**29 for x in s[:8])
+        d = lc_finish29([4 * pi + a - b for pi, a, b in zip(limbs29(m), y0, y1)], m)
+        dv = sum(v << (29 * i) for i, v in enumerate(d))
+        assert dv < 2 * m and dv % m == (v0 - v1) % m
+        op = [a + r - b for a, r, b in zip(y0, c["p8r"], y1)]
+        assert all(0 <= x < 5 * 2**29 for x in op)
+        w = rng.choice(tw)
+        t, top = mont29_limbs(op, w, m)
+        wv = sum(v << (29 * i) for i, v in enumerate(w))
+        assert top < 2**64 and t < 2 * m
+        assert t % m == (v0 - v1) * wv * pow(2, -261, m) % m
 
 
 @pytest.mark.parametrize("m", [R, P, 2**255 - 19, 2**255 - 1, 3, 2**64 + 13])

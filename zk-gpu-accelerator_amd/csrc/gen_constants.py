@@ -17,11 +17,13 @@ def limbs29(v: int) -> list[int]:
     return [(v >> (LIMB_BITS * i)) & ((1 << LIMB_BITS) - 1) for i in range(NLIMBS)]
 
 
-def redundant29(v: int) -> list[int]:
-    """v in 29-bit limbs, each of limbs 0..7 raised by 2^29 (borrowed from the next limb), so that
-    a + v - b stays non-negative limb by limb for normalized a and b < v / 2."""
+def redundant29(v: int, lift: int = 1) -> list[int]:
+    """v in 29-bit limbs, each of limbs 0..7 raised by lift * 2^29 (borrowed from the next limb), so
+    that a + v - b stays non-negative limb by limb for a b whose limbs 0..7 are at most
+    lift * 2^29 - lift (normalized b for lift 1, a sum of two normalized values for lift 2)."""
     l = limbs29(v)
-    r = [l[0] + (1 << LIMB_BITS)] + [l[i] + (1 << LIMB_BITS) - 1 for i in range(1, NLIMBS - 1)] + [l[NLIMBS - 1] - 1]
+    up = lift << LIMB_BITS
+    r = [l[0] + up] + [l[i] + up - lift for i in range(1, NLIMBS - 1)] + [l[NLIMBS - 1] - lift]
     assert sum(x << (LIMB_BITS * i) for i, x in enumerate(r)) == v and r[-1] >= 0
     return r
 
@@ -31,6 +33,7 @@ def field_constants(m: int) -> dict[str, list[int] | int]:
         "p": limbs29(m),
         "p2": limbs29(2 * m),              # 2p, the bound of lazily reduced values
         "p4r": redundant29(4 * m),         # 4p with every limb below the top at least 2^29 - 1
+        "p8r": redundant29(8 * m, 2),      # 8p with every limb below the top at least 2^30 - 2
         "ninv": (-pow(m, -1, 1 << LIMB_BITS)) % (1 << LIMB_BITS),
         "c256": limbs29(pow(2, 256, m)),   # multiplier taking a 2^261 form to the 2^256 form
         "c266": limbs29(pow(2, 266, m)),   # multiplier taking the 2^256 form to the 2^261 form
@@ -44,7 +47,7 @@ def field_constants(m: int) -> dict[str, list[int] | int]:
 def emit(name: str, m: int) -> str:
     c = field_constants(m)
     out = [f"struct {name} {{", f"  // modulus {m:#x}"]
-    for key in ("p", "p2", "p4r", "c256", "c266", "c517", "c522", "one"):
+    for key in ("p", "p2", "p4r", "p8r", "c256", "c266", "c517", "c522", "one"):
         vals = ", ".join(f"0x{v:08x}u" for v in c[key])
         out.append(f"  static constexpr uint32_t {key}[9] = {{{vals}}};")
     out.append(f"  static constexpr uint32_t p32[8] = {{{', '.join(f'0x{v:08x}u' for v in c['p32'])}}};")

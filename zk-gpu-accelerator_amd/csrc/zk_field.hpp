@@ -442,9 +442,10 @@ __device__ __forceinline__ Fe lc_carry(Acc64 a) {
   return r;
 }
 
-// The value V (0 <= V < 2^256, limbs below 2^36 in magnitude) less q p, q = floor(t / (p_8 + 1))
+// The value V (0 <= V < 2^261, limbs below 2^36 in magnitude) less q p, q = floor(t / (p_8 + 1))
 // for t = the top limb less a margin for the uncarried lower limbs (they move V / 2^232 by less
-// than 2^8): q p <= V, and V - q p < p (1 + 2^13 / p_8) < 2p for the BN254 moduli (p_8 > 2^21).
+// than 2^8): q p <= V, and V - q p < 2^232 (p_8 + 4096 + 2^8 + V / (2^232 (p_8 + 1)) + 1)
+// < p (1 + 2^13 / p_8) < 2p for the BN254 moduli (p_8 > 2^21).
 template <class F>
 __device__ __forceinline__ Fe lc_finish(const F&, Acc64 a) {
   static_assert(F::p[NL - 1] > (1u << 21), "the quotient estimate needs a full-size top limb");
@@ -453,6 +454,47 @@ __device__ __forceinline__ Fe lc_finish(const F&, Acc64 a) {
 #pragma unroll
   for (int i = 0; i < NL; ++i) a.l[i] -= (int64_t)q * (int64_t)F::p[i];
   return lc_carry(a);
+}
+
+// Sums kept wide between two butterfly stages (built-in fields, which carry p8r: 8p with limbs 1..7
+// raised by 2^30 - 2 and limb 0 by 2^30, the borrows taken from the limb above).
+// a + b without carries: for a, b < 2p with normalized limbs, a value below 4p with limbs
+// 0..7 at most 2^30 - 2.
+__device__ __forceinline__ Fe fadd_wide(const Fe& a, const Fe& b) {
+  Fe s;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) s.l[i] = a.l[i] + b.l[i];
+  return s;
+}
+
+// The operand of a product: a - b + 8p in (4p, 12p) with no carries, for a, b from fadd_wide.
+// Against p8r each limb a_i + p8r_i - b_i lies in [0, 5 * 2^29), so against a normalized operand
+// below 2p a column of mont_raw sums at most 9 * 5 * 2^58 + 9 * 2^58 + 2^35 < 2^64, and the result
+// (a b + m p) / 2^261 < 24 p^2 / 2^261 + p stays below 2p.
+template <class F>
+__device__ __forceinline__ Fe fsub_wide(const F&, const Fe& a, const Fe& b) {
+  Fe d;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) d.l[i] = a.l[i] + F::p8r[i] - b.l[i];
+  return d;
+}
+
+// a + b below 2p with normalized limbs, for a, b from fadd_wide (a sum below 8p, limbs below 2^31).
+template <class F>
+__device__ __forceinline__ Fe fadd_wide_finish(const F& f, const Fe& a, const Fe& b) {
+  Acc64 t;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) t.l[i] = (int64_t)(a.l[i] + b.l[i]);
+  return lc_finish(f, t);
+}
+
+// a - b in [0, 2p) with normalized limbs, for a, b from fadd_wide (a - b + 4p lies in (0, 8p)).
+template <class F>
+__device__ __forceinline__ Fe fsub_wide_finish(const F& f, const Fe& a, const Fe& b) {
+  Acc64 t = lc_start(f, 4);
+#pragma unroll
+  for (int i = 0; i < NL; ++i) t.l[i] += (int64_t)a.l[i] - (int64_t)b.l[i];
+  return lc_finish(f, t);
 }
 
 template <class F>

@@ -22,8 +22,11 @@
 // 2p, a difference u - v + 4p (limbs kept redundant, no carries) goes straight
 // into the product, whose result is below 2p with no final subtraction
 // (zk_field.hpp, fsub_for_mul); intermediate passes store such values,
-// the last pass stores canonical ones. The last stage of a transform has the
-// single twiddle 1 and does no product.
+// the last pass stores canonical ones. In a pair of stages done in registers
+// the first stage's sums stay wide (below 4p, limbs without carries) and are
+// reduced once, by the second stage: its sum by a quotient estimated from the
+// top limb, its difference + 8p inside the product (fsub_wide). The last stage
+// of a transform has the single twiddle 1 and does no product.
 //
 // Twiddles. One table per stage, w_s[j] = omega^(j * 2^s), so that a pass
 // reads each stage's twiddles contiguously; values are in the 261-form, so
@@ -157,18 +160,33 @@ __device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64
       const int sl0 = (m00 << geo.log_c) + g, step = h2 << geo.log_c;
       const Fe x0 = lds_get(lds, sl0), x1 = lds_get(lds, sl0 + step), x2 = lds_get(lds, sl0 + 2 * step),
                x3 = lds_get(lds, sl0 + 3 * step);
-      const Fe y0 = fadd_lazy(f, x0, x2), y1 = fadd_lazy(f, x1, x3);
       const Fe y2 = mont_raw(f, fsub_for_mul(f, x0, x2), load_tw(tw_a, ja));
       const Fe y3 = mont_raw(f, fsub_for_mul(f, x1, x3), load_tw(tw_a, jb));
-      lds_put(lds, sl0, fadd_lazy(f, y0, y1));
       lds_put(lds, sl0 + 2 * step, fadd_lazy(f, y2, y3));
-      if (last_b) {
-        lds_put(lds, sl0 + step, fsub_lazy(f, y0, y1));
-        lds_put(lds, sl0 + 3 * step, fsub_lazy(f, y2, y3));
+      if constexpr (HasP4r<F>::value) {
+        // the first stage's sums stay wide (below 4p, no carries) and are reduced once, in the
+        // second stage's sum, difference or product
+        const Fe y0 = fadd_wide(x0, x2), y1 = fadd_wide(x1, x3);
+        lds_put(lds, sl0, fadd_wide_finish(f, y0, y1));
+        if (last_b) {
+          lds_put(lds, sl0 + step, fsub_wide_finish(f, y0, y1));
+          lds_put(lds, sl0 + 3 * step, fsub_lazy(f, y2, y3));
+        } else {
+          const Fe wc = load_tw(tw_b, ja);
+          lds_put(lds, sl0 + step, mont_raw(f, fsub_wide(f, y0, y1), wc));
+          lds_put(lds, sl0 + 3 * step, mont_raw(f, fsub_for_mul(f, y2, y3), wc));
+        }
       } else {
-        const Fe wc = load_tw(tw_b, ja);
-        lds_put(lds, sl0 + step, mont_raw(f, fsub_for_mul(f, y0, y1), wc));
-        lds_put(lds, sl0 + 3 * step, mont_raw(f, fsub_for_mul(f, y2, y3), wc));
+        const Fe y0 = fadd_lazy(f, x0, x2), y1 = fadd_lazy(f, x1, x3);
+        lds_put(lds, sl0, fadd_lazy(f, y0, y1));
+        if (last_b) {
+          lds_put(lds, sl0 + step, fsub_lazy(f, y0, y1));
+          lds_put(lds, sl0 + 3 * step, fsub_lazy(f, y2, y3));
+        } else {
+          const Fe wc = load_tw(tw_b, ja);
+          lds_put(lds, sl0 + step, mont_raw(f, fsub_for_mul(f, y0, y1), wc));
+          lds_put(lds, sl0 + 3 * step, mont_raw(f, fsub_for_mul(f, y2, y3), wc));
+        }
       }
     }
     __syncthreads();
