@@ -162,8 +162,8 @@ __device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64
                x3 = lds_get(lds, sl0 + 3 * step);
       const Fe y2 = mont_raw(f, fsub_for_mul(f, x0, x2), load_tw(tw_a, ja));
       const Fe y3 = mont_raw(f, fsub_for_mul(f, x1, x3), load_tw(tw_a, jb));
-      lds_put(lds, sl0 + 2 * step, fadd_lazy(f, y2, y3));
       if constexpr (HasP4r<F>::value) {
+        lds_put(lds, sl0 + 2 * step, fadd_wide_finish(f, y2, y3));
         // the first stage's sums stay wide (below 4p, no carries) and are reduced once, in the
         // second stage's sum, difference or product
         const Fe y0 = fadd_wide(x0, x2), y1 = fadd_wide(x1, x3);
@@ -177,6 +177,7 @@ __device__ __forceinline__ void ntt_tile(const F& f, const NttPassDesc& d, int64
           lds_put(lds, sl0 + 3 * step, mont_raw(f, fsub_for_mul(f, y2, y3), wc));
         }
       } else {
+        lds_put(lds, sl0 + 2 * step, fadd_lazy(f, y2, y3));
         const Fe y0 = fadd_lazy(f, x0, x2), y1 = fadd_lazy(f, x1, x3);
         lds_put(lds, sl0, fadd_lazy(f, y0, y1));
         if (last_b) {
