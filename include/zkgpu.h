@@ -178,7 +178,8 @@ int zk_msm_set_accumulate(int variant);
 int zk_msm_get_accumulate(int* variant);
 /* log2 of the bucket-reduction segment length L of plans created afterwards (picked by the
  * autotuner): each thread forms the running sums of L consecutive buckets, the segments' sums
- * then meet in log2(buckets / L) bit rows; 0 = automatic (L = 4). The reference's segments are a
+ * then meet in log2(buckets / L) bit rows; 0 = automatic (L = 4 for windows of 14 bits and
+ * more, else L = 2). The reference's segments are a
  * fixed length and their sums are scaled one by one (msm_gpu.py:196-223). */
 int zk_msm_set_segment(int log_seg);
 int zk_msm_get_segment(int* log_seg);

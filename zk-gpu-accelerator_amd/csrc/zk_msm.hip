@@ -1150,20 +1150,32 @@ __global__ void __launch_bounds__(64 * kWinGroups) k_window_total(int64_t total_
   if (threadIdx.x == 0) store_jac_rec(wsum + g, r);
 }
 
-// One wavefront per job: Horner over the windows, sum_w 2^(cw) U_w, from the top window down, with
-// the limb-sliced formulas (the chain's latency is the doubling's).
-__global__ void __launch_bounds__(64) k_final(const MsmJob* __restrict__ jobs, int njobs, const JacRec* __restrict__ wsum) {
+// One workgroup per job: sum_w 2^(cw) U_w with the limb-sliced formulas. The windows split into
+// kFinalWaves contiguous groups, the top one the smallest; wavefront g runs Horner over its group
+// [lo, hi) from the top down, then doubles its sum c * lo times, and the groups' results meet in a
+// two-level tree. Every path keeps the (rw - 1) c doublings a single Horner chain has (the top
+// window's), but only the top group's additions and the tree's stay on it: at 2^10 points (22
+// windows of 6 bits) 5 additions against 21.
+constexpr int kFinalWaves = 4;
+
+__global__ void __launch_bounds__(64 * kFinalWaves) k_final(const MsmJob* __restrict__ jobs, int njobs,
+                                                            const JacRec* __restrict__ wsum) {
+  __shared__ JacRec sh[kFinalWaves];
   const int jb = blockIdx.x;
   if (jb >= njobs) return;
   const MsmJob& job = jobs[jb];
   const FqConst f;
   const SlLane c = sl_lane<FqConst>();
-  SlJac acc = sl_from_jac(job.rw > 0 ? load_jac_rec(wsum + job.win_base + job.rw - 1) : jac_identity(f));
-  for (int w = job.rw - 2; w >= 0; --w) {
+  const int per = (job.rw + kFinalWaves - 1) / kFinalWaves;
+  const int lo = min(job.rw, ((int)threadIdx.x >> 6) * per), hi = min(job.rw, lo + per);
+  SlJac acc = sl_from_jac(hi > lo ? load_jac_rec(wsum + job.win_base + hi - 1) : jac_identity(f));
+  for (int w = hi - 2; w >= lo; --w) {
     for (int e = 0; e < job.c; ++e) acc = sl_dbl<FqConst>(c, acc);
     acc = sl_add<FqConst>(c, acc, sl_from_jac(load_jac_rec(wsum + job.win_base + w)));
   }
-  Jac r = sl_to_jac<FqConst>(c, acc);
+  if (hi > lo)
+    for (int e = job.c * lo; e > 0; --e) acc = sl_dbl<FqConst>(c, acc);
+  Jac r = sl_to_jac<FqConst>(c, sl_workgroup_sum<kFinalWaves>(sh, c, acc));
   if (fe_is_zero_lazy(f, r.z)) r = jac_identity(f);
   if (threadIdx.x == 0) store_jac(f, job.out, r);
 }
@@ -1192,7 +1204,10 @@ __global__ void __launch_bounds__(256) k_srs_window_table(const AffRec* __restri
 // Whether new plans split scalars with the GLV endomorphism (zk_msm_set_glv).
 static int g_glv = 1;
 
-// log2 of the reduction's segment length for new plans (zk_msm_set_segment); 0: min(2, c - 1).
+// log2 of the reduction's segment length for new plans (zk_msm_set_segment); 0: automatic, 2
+// for windows of 14 bits and more and 1 below (min c - 1). Narrow windows come with small MSMs,
+// whose reduction is latency-bound: two-bucket leaves halve its first chain (2^10..2^17 points:
+// 5% faster per MSM), while 2^18 and 2^20 points are 5% faster with four-bucket leaves.
 static int g_log_seg = 0;
 
 // The bucket accumulation kernel new plans use (zk_msm_set_accumulate): bit 0 picks the
@@ -1278,7 +1293,7 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
     j.nv = j.glv ? 2 * j.n : j.n;
     j.windows = (j.glv ? Glv::kBits : 254) / j.c + 1;
     j.rw = j.table ? 1 : j.windows;
-    j.log_seg = std::min(g_log_seg > 0 ? g_log_seg : 2, j.c - 1);
+    j.log_seg = std::min(g_log_seg > 0 ? g_log_seg : (j.c >= 14 ? 2 : 1), j.c - 1);
     j.log_segs = (j.c - 1) - j.log_seg;
     // chunks per reduction row: a row adds half the segments, or (the first T half) ceil(segs / 2)
     j.chunks = (int)((((1LL << j.log_segs) + 1) / 2 + kRedChunk - 1) / kRedChunk);
@@ -1531,7 +1546,7 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
                        plan->windows_total, plan->wparts, plan->wsum);
     ZK_LAUNCH_CHECK("msm window sums");
   }
-  hipLaunchKernelGGL(k_final, dim3((unsigned)nj), dim3(64), 0, stream, plan->d_jobs, nj, plan->wsum);
+  hipLaunchKernelGGL(k_final, dim3((unsigned)nj), dim3(64 * kFinalWaves), 0, stream, plan->d_jobs, nj, plan->wsum);
   ZK_LAUNCH_CHECK("msm final");
   return 0;
 }
