@@ -782,7 +782,14 @@ __device__ __forceinline__ void close_runs(const RunSummary* sm, BucketRec* __re
 }
 
 constexpr int kAccThreads = 64;      // one wave per accumulation workgroup
-constexpr int kCombineThreads = 256;
+constexpr int kCombineThreads = 256;  // summaries per combine workgroup
+// Threads of a combine workgroup: the tree's upper levels pick their mode from the workgroup's rows
+// and wavefronts, so 512 threads (32 rows, 8 wavefronts) run the level of 32 merges one per row and
+// the last four one per wavefront (limb-sliced) instead of one per thread and per row. At 228
+// registers a CU then holds one such workgroup instead of two, so levels with more workgroups than
+// CUs keep 256 threads (1024 threads would cap the registers at 128, below the thread-serial
+// addition's).
+constexpr int kCombineLanes = 512;
 
 // Level 0, a persistent grid: the m sorted entries are split evenly, ceil(m / threads) consecutive
 // entries per thread; complete runs leave as bucket records in the accumulator's coordinates, the
@@ -951,16 +958,18 @@ __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in
 
 // The `count` summaries of the previous level, 256 per workgroup, merged by a tree (empty
 // summaries are not copied).
-__global__ void __launch_bounds__(kCombineThreads) k_combine(const RunSummary* __restrict__ in, int64_t count,
-                                                             BucketRec* __restrict__ buckets, RunSummary* __restrict__ out,
-                                                             int last_level) {
+__global__ void __launch_bounds__(kCombineLanes) k_combine(const RunSummary* __restrict__ in, int64_t count,
+                                                           BucketRec* __restrict__ buckets, RunSummary* __restrict__ out,
+                                                           int last_level) {
   __shared__ RunSummary sh[kCombineThreads];
-  const int64_t t = (int64_t)blockIdx.x * kCombineThreads + threadIdx.x;
-  const uint32_t flags = t < count ? in[t].flags : 0u;
-  if (flags) {
-    sh[threadIdx.x] = in[t];
-  } else {
-    sh[threadIdx.x].flags = 0;
+  if (threadIdx.x < kCombineThreads) {
+    const int64_t t = (int64_t)blockIdx.x * kCombineThreads + threadIdx.x;
+    const uint32_t flags = t < count ? in[t].flags : 0u;
+    if (flags) {
+      sh[threadIdx.x] = in[t];
+    } else {
+      sh[threadIdx.x].flags = 0;
+    }
   }
   const int64_t valid = min((int64_t)kCombineThreads, count - (int64_t)blockIdx.x * kCombineThreads);
   int width = 1;
@@ -1525,7 +1534,8 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
         ZK_LAUNCH_CHECK("msm boundaries");
         cur ^= 1;
       }
-      hipLaunchKernelGGL(k_combine, dim3((unsigned)plan->level_max[(size_t)l]), dim3(kCombineThreads), 0, stream,
+      const int lanes = plan->level_max[(size_t)l] <= device_cus() ? kCombineLanes : kCombineThreads;
+      hipLaunchKernelGGL(k_combine, dim3((unsigned)plan->level_max[(size_t)l]), dim3(lanes), 0, stream,
                          plan->runs[cur], plan->level_max[(size_t)l - 1], plan->buckets, plan->runs[cur ^ 1],
                          l + 1 == levels ? 1 : 0);
       ZK_LAUNCH_CHECK("msm combine");
