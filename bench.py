@@ -87,6 +87,8 @@ def main() -> int:
     ap.add_argument("--log-ntt", type=int, default=22)
     ap.add_argument("--no-extras", action="store_true", help="only the headline MSM")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--ntt-only", action="store_true",
+                    help="only the forward NTT at --log-ntt (for the pass kernel's trace and byte counters)")
     ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs of the served workload")
     ap.add_argument("--serve-rate", type=float, default=300.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serve-budget-gib", type=float, default=24.0, help="HBM budget of the server")
@@ -110,6 +112,13 @@ def main() -> int:
     gb.require_gpu("bench")
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
 
+    if args.ntt_only:
+        if args.traffic_probe:
+            _copy_probe(gb)
+        gb.prof_enable(True)
+        print(json.dumps(_ntt_and_pipeline(args, gb, BN254_FR, workloads, log, quotient=False)), flush=True)
+        return 0
+
     # ---- headline: plain MSM at 2^20 --------------------------------------
     n = 1 << args.log_msm
     log(f"setup: basis of {n} points k_i G on the GPU, witness-like scalars")
@@ -124,11 +133,7 @@ def main() -> int:
         msm_g1_jacobian(points, scalars, out, window_size=window)
 
     if args.traffic_probe:
-        probe = 256 << 20
-        a, b = gb.empty(probe // 8), gb.empty(probe // 8)
-        gb.call("zk_copy", b.ptr, a.ptr, probe, gb.current_stream())
-        gb.synchronize()
-        del a, b
+        _copy_probe(gb)
     log("msm: warmup + timed")
     gb.prof_enable(True)
     for _ in range(args.warmup):
@@ -155,7 +160,7 @@ def main() -> int:
     log(f"msm 2^{args.log_msm}: {ms:.3f} ms/step, {msm_mpts:.2f} Mpts/s; accumulate {acc_avg:.3f} ms "
         f"({achieved_tmads:.2f} Tmad/s of {peak_tmads:.2f})")
 
-    traffic = _traffic("k_accumulate", n)
+    traffic = _traffic("k_accumulate", "msm_points", n)
     extras: dict = {}
     if not args.no_extras:
         extras.update(_msm_2p16(args, gb, workloads, msm_g1_jacobian, log))
@@ -224,13 +229,23 @@ def _msm_2p16(args, gb, workloads, msm_g1_jacobian, log) -> dict:
     return {"msm_2p16": {"ms": round(ms, 4), "mpts_s": round(n / (ms * 1e-3) / 1e6, 3)}}
 
 
-def _traffic(kernel: str, n: int) -> dict | None:
-    """HBM bytes per launch of `kernel` from the committed byte-counter summary, if one exists."""
+def _copy_probe(gb) -> None:
+    """256 MiB through zk_copy: the known-size kernel that scales the byte counters."""
+    probe = 256 << 20
+    a, b = gb.empty(probe // 8), gb.empty(probe // 8)
+    gb.call("zk_copy", b.ptr, a.ptr, probe, gb.current_stream())
+    gb.synchronize()
+    del a, b
+
+
+def _traffic(kernel: str, key: str, size: int) -> dict | None:
+    """HBM bytes per launch of `kernel` from the committed byte-counter summary, if one exists
+    for this workload size (`key` names the size field)."""
     path = ROOT / "profiles" / f"{kernel}_traffic.json"
     if not path.exists():
         return None
     data = json.loads(path.read_text())
-    if data.get("msm_points") != n:
+    if data.get(key) != size:
         return None
     return {"bytes_per_launch": data["bytes_per_launch"], "source": f"profiles/{path.name}"}
 
@@ -252,7 +267,7 @@ def _valu_peak(gb) -> float:
     return float(out.value)
 
 
-def _ntt_and_pipeline(args, gb, Fr, workloads, log) -> dict:
+def _ntt_and_pipeline(args, gb, Fr, workloads, log, quotient: bool = True) -> dict:
     n = 1 << args.log_ntt
     log(f"ntt: 2^{args.log_ntt} forward, uniform Fr rows")
     x = Fr.from_limbs(workloads.uniform_rows(n, 0x5EED03))
@@ -282,17 +297,21 @@ def _ntt_and_pipeline(args, gb, Fr, workloads, log) -> dict:
     log(f"ntt 2^{args.log_ntt}: {ms:.3f} ms, {melem:.1f} Melem/s; pass {per_pass:.3f} ms ({pass_gbps:.0f} GB/s, "
         f"{ntt_tmads:.2f} Tmad/s of {peak:.2f})")
 
-    log("groth16 quotient pipeline at the same size (7 NTTs)")
-    a, b, c = (Fr.from_limbs(workloads.uniform_rows(n, s)) for s in (0x5EED31, 0x5EED32, 0x5EED33))
-    h = gb.empty((n, 4))
-    shift = gb.limbs_of(7)
+    traffic = _traffic("k_ntt_pass", "ntt_log_n", args.log_ntt)
+    extras = {}
+    if quotient:
+        log("groth16 quotient pipeline at the same size (7 NTTs)")
+        a, b, c = (Fr.from_limbs(workloads.uniform_rows(n, s)) for s in (0x5EED31, 0x5EED32, 0x5EED33))
+        h = gb.empty((n, 4))
+        shift = gb.limbs_of(7)
 
-    def pipe_step() -> None:
-        gb.call("zk_groth16_quotient", a.ptr, b.ptr, c.ptr, h.ptr, args.log_ntt, shift, gb.current_stream())
+        def pipe_step() -> None:
+            gb.call("zk_groth16_quotient", a.ptr, b.ptr, c.ptr, h.ptr, args.log_ntt, shift, gb.current_stream())
 
-    pipe_step()
-    pms = timed(pipe_step, max(1, args.steps // 2), 1, gb)
-    log(f"groth16 2^{args.log_ntt}: {pms:.3f} ms")
+        pipe_step()
+        pms = timed(pipe_step, max(1, args.steps // 2), 1, gb)
+        log(f"groth16 2^{args.log_ntt}: {pms:.3f} ms")
+        extras["groth16_quotient"] = {"log_n": args.log_ntt, "ms": round(pms, 4), "coset_shift": 7}
     return {
         "ntt": {
             "log_n": args.log_ntt,
@@ -311,10 +330,13 @@ def _ntt_and_pipeline(args, gb, Fr, workloads, log) -> dict:
                 "work_per_unit": "one butterfly: 1 Fr mul x 136 32-bit mad; n/2 butterflies x log n stages",
                 "hbm": {"achieved": round(pass_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                         "frac": round(pass_gbps / HBM_PEAK_GBPS, 4), "bytes_per_pass": 64 * n},
-                "traffic": None,
+                "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                "traffic_source": traffic.get("source") if traffic else None,
+                "algorithmic_bytes_per_launch": 64 * n,
+                "avg_launch_ms": round(per_pass, 4),
             },
         },
-        "groth16_quotient": {"log_n": args.log_ntt, "ms": round(pms, 4), "coset_shift": 7},
+        **extras,
     }
 
 
