@@ -2,8 +2,8 @@
 //
 // Computes sum_i s_i * P_i with the bucket method, entirely on the device:
 //
-//  1. prep     points -> 261-form coordinates packed in 80-byte records, plus
-//              phi(P_i) = (beta x_i, y_i) = lambda P_i, the curve's endomorphism.
+//  1. prep     points -> 64-byte records (two 32-byte rows of 261-form coordinates),
+//              plus phi(P_i) = (beta x_i, y_i) = lambda P_i, the curve's endomorphism.
 //  2. digits   every scalar (reduced mod r) is split k = k1 + k2 lambda with
 //              |k1|, |k2| < 2^127 (GLV: a lattice basis of {(a, b): a + b lambda
 //              = 0 mod r}, zk_constants.hpp), and |k1| (for P_i) and |k2| (for
@@ -12,18 +12,17 @@
 //              digit d != 0 sends +-P to bucket (window, |d|). Against the
 //              reference's unsigned digits of the full scalar (src/zkgpu/msm_gpu.py
 //              :89-105) this halves the buckets per window and the windows, so the
-//              bucket reduction and the final Horner chain are a quarter and a half
+//              bucket reduction and the final combination are a quarter and a half
 //              as long; the entries grow from W254 n to 2 W127 n (about +12%).
-//  3. sort     counting sort of the (window, bucket) keys with no global atomics:
-//              digits are written window-major; one workgroup per (window, range
-//              of 16384 buckets, tile of scalars) counts its tile's digits into an
-//              LDS histogram and writes it as one contiguous row; one pass down
-//              the tiles of each bucket turns the counts into per-tile offsets and
-//              a bucket total, a scan of the totals gives every bucket its start,
-//              and the same workgroups then place packed (key, point index | sign)
-//              records from LDS cursors. Witness-like scalars put a large share of
-//              all entries into a few buckets (booleans send ~15% of them to
-//              bucket 1 of window 0); those hot keys only ever meet LDS atomics.
+//  3. sort     two counting sorts with no global atomics, so that writes stay local.
+//              Pass A sorts the digits by partition (a window's buckets in groups of
+//              512): LDS histograms per (window, tile of scalars), per-partition
+//              offsets, and placement of packed (key, point index | sign) records
+//              from LDS cursors, in runs of hundreds. Pass B sorts each partition by
+//              bucket in chunks of 8192 records, staged and sorted in LDS and written
+//              one bucket run at a time. Witness-like scalars put a large share of all
+//              entries into a few buckets (booleans send ~15% of them to bucket 1 of
+//              window 0); those hot keys only ever meet LDS atomics.
 //  4. buckets  a persistent grid splits the m sorted entries evenly, ceil(m / threads)
 //              consecutive entries per thread, added to an XYZZ accumulator with
 //              affine points (madd-2008-s: 10 products instead of Jacobian's 11);
@@ -31,23 +30,25 @@
 //              bucket and is written at once, in the accumulator's coordinates (no
 //              conversion inside the loop: some lane of a wave ends a run in most
 //              iterations, so a conversion there would cost the whole wave 4 products
-//              per entry; the reduction converts each bucket once). The runs cut by range
-//              boundaries leave as one run summary per thread and are merged by trees
-//              of 256 summaries per workgroup, level after level, each merge at most
-//              one addition. The load
-//              is balanced whatever the bucket sizes, with no host sync (the
-//              reference splits buckets into chunks level by level and syncs the
-//              host per level, msm_gpu.py:287-333).
-//  5. reduce   per window, sum_v v * B_v: threads take segments of L buckets (each
-//              converted to Jacobian coordinates as it is loaded) with
-//              running sums (T = sum (i+1) B, R = sum B); a window's total is
-//              sum_s T_s + L * sum_j 2^j Q_j with Q_j = sum of R_s over segments s
-//              whose index has bit j set, each Q_j (and sum T_s, in two halves)
-//              block-wide tree sums with the same number of additions per block.
+//              per entry). The runs cut by range boundaries leave as one run summary
+//              per thread; boundary passes join the fragments of short chains, and
+//              trees of 256 summaries per workgroup merge the rest, level after level,
+//              each merge at most one addition. The load is balanced whatever the
+//              bucket sizes, with no host sync (the reference splits buckets into
+//              chunks level by level and syncs the host per level,
+//              msm_gpu.py:287-333).
+//  5. reduce   per window, sum_v v * B_v: threads take segments of L buckets (2 or 4,
+//              read in XYZZ coordinates) with running sums (T = sum (i+1) B,
+//              R = sum B); a window's total is sum_s T_s + L * sum_j 2^j Q_j with
+//              Q_j = sum of R_s over segments s whose index has bit j set, each Q_j
+//              (and sum T_s, in two halves) a block-wide tree sum with the same number
+//              of additions per block.
 //  6. combine  per window, the terms 2^(j + log L) Q_j are doubled side by side and
-//              summed; a Horner chain over the windows then gives the Jacobian
-//              result on the device, both with wave-parallel formulas (the reference
-//              combines windows on the CPU, msm_gpu.py:449 and msm.py:156-169).
+//              summed; the windows then meet in sum_w 2^(cw) U_w, Horner over four
+//              groups of windows side by side, all with the limb-sliced formulas of
+//              zk_curve_sliced.hpp, and the Jacobian result stays on the device (the
+//              reference combines windows on the CPU, msm_gpu.py:449 and
+//              msm.py:156-169).
 //
 // The result is a unique group element, so it equals the reference's
 // msm_pippenger for every input; intermediate coordinates never leave the device.
