@@ -165,6 +165,7 @@ def main() -> int:
     if not args.no_extras:
         extras.update(_msm_2p16(args, gb, workloads, msm_g1_jacobian, log))
         extras.update(_ntt_and_pipeline(args, gb, BN254_FR, workloads, log))
+        extras.update(_field_mul(args, gb, BN254_FR, workloads, log))
         from zkgpu.batch import SRS
 
         log("srs: device-resident basis with window tables of widths 12 and 16 (setup, not timed)")
@@ -265,6 +266,40 @@ def _valu_peak(gb) -> float:
     out = ctypes.c_double(0)
     gb.call("zk_valu_peak", ctypes.byref(out))
     return float(out.value)
+
+
+def _field_mul(args, gb, Fr, workloads, log) -> dict:
+    """Element-wise Fr Montgomery products over 2^24 device-resident rows (the reference's
+    Field256.mul benchmark, field256_gpu.py:325-341, at a size that fills the GPU): 64 bytes in
+    and 32 out per product, so the kernel is bound by HBM, with its VALU rate beside it (two
+    29-bit-limb Montgomery products per element, since the rows keep the reference's 2^256 form:
+    a b 2^-256 = mont(a, mont(b, 2^266)))."""
+    n = 1 << 24
+    log("field: 2^24 Fr products, uniform rows")
+    a = Fr.from_limbs(workloads.uniform_rows(n, 0x5EED41))
+    b = Fr.from_limbs(workloads.uniform_rows(n, 0x5EED42))
+    out = gb.empty((n, 4))
+    fid = Fr._compile()["id"]
+
+    def mul_step() -> None:
+        gb.call("zk_field_mul", fid, a.ptr, 0, b.ptr, 0, out.ptr, n, gb.current_stream())
+
+    ms = timed(mul_step, args.steps, 2, gb)
+    gbps = 96.0 * n / (ms * 1e-3) / 1e9
+    gmul = n / (ms * 1e-3) / 1e9
+    tmads = n * 2 * 136 / (ms * 1e-3) / 1e12
+    log(f"field mul 2^24: {ms:.3f} ms, {gmul:.2f} G mul/s, {gbps:.0f} GB/s, {tmads:.2f} Tmad/s")
+    del a, b, out
+    return {
+        "field_mul": {
+            "n": n,
+            "ms": round(ms, 4),
+            "gmul_s": round(gmul, 3),
+            "roofline": {"kernel": "k_binary<mul>", "bound": "hbm", "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS,
+                         "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                         "algorithmic_bytes_per_launch": 96 * n, "valu_tmad_s": round(tmads, 3)},
+        }
+    }
 
 
 def _ntt_and_pipeline(args, gb, Fr, workloads, log, quotient: bool = True) -> dict:

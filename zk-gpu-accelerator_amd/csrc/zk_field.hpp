@@ -606,10 +606,13 @@ __device__ __forceinline__ Fe form261_to_256(const F& f, const Fe& x) {
   return mont(f, x, fe_const<F>(f.c256));
 }
 
-// a * b in the 256-form for two 256-form operands: a * (b * 2^5) * 2^-261.
+// a * b in the 256-form for two 256-form operands: (a * b * 2^-261) * 2^5, the second factor as
+// form256_to_261's shift and quotient for the built-in moduli (one product instead of two), else
+// a * (b * 2^5) * 2^-261.
 template <class F>
 __device__ __forceinline__ Fe mul256(const F& f, const Fe& a, const Fe& b) {
-  return mont(f, a, mont(f, b, fe_const<F>(f.c266)));
+  if constexpr (HasP4r<F>::value) return form256_to_261(f, mont_raw(f, a, b));  // below 2p < 2^256
+  else return mont(f, a, mont(f, b, fe_const<F>(f.c266)));
 }
 
 // x^e for a 261-form x and a 256-bit exponent (plain words, little-endian), in the 261-form.
