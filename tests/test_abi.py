@@ -69,6 +69,7 @@ def test_constants(struct, m):
     assert c["ninv"] == (-pow(m, -1, 2**29)) % 2**29
     for name, e in (("c256", 256), ("c266", 266), ("c517", 517), ("c522", 522), ("one", 261)):
         assert c[name] == limbs29(pow(2, e, m)), name
+    assert c["mone"] == limbs29(-pow(2, 261, m) % m)  # -1 in the 2^261 form
     assert c["p32"] == [(m >> (32 * i)) & 0xFFFFFFFF for i in range(8)]
     # redundant 4p and 8p: the value, and limbs 0..7 large enough that a + kp - b keeps every
     # limb non-negative for a normalized b (4p) or a carry-free sum of two normalized values (8p)

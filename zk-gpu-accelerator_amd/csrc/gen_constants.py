@@ -40,6 +40,7 @@ def field_constants(m: int) -> dict[str, list[int] | int]:
         "c517": limbs29(pow(2, 517, m)),   # multiplier taking a plain value to the 2^256 form
         "c522": limbs29(pow(2, 522, m)),   # multiplier taking a plain value to the 2^261 form
         "one": limbs29(pow(2, RBITS, m)),  # 1 in the 2^261 form
+        "mone": limbs29(-pow(2, RBITS, m) % m),  # -1 in the 2^261 form
         "p32": [(m >> (32 * i)) & 0xFFFFFFFF for i in range(8)],
     }
 
@@ -47,7 +48,7 @@ def field_constants(m: int) -> dict[str, list[int] | int]:
 def emit(name: str, m: int) -> str:
     c = field_constants(m)
     out = [f"struct {name} {{", f"  // modulus {m:#x}"]
-    for key in ("p", "p2", "p4r", "p8r", "c256", "c266", "c517", "c522", "one"):
+    for key in ("p", "p2", "p4r", "p8r", "c256", "c266", "c517", "c522", "one", "mone"):
         vals = ", ".join(f"0x{v:08x}u" for v in c[key])
         out.append(f"  static constexpr uint32_t {key}[9] = {{{vals}}};")
     out.append(f"  static constexpr uint32_t p32[8] = {{{', '.join(f'0x{v:08x}u' for v in c['p32'])}}};")

@@ -52,22 +52,29 @@ __device__ __forceinline__ Xyzz xyzz_dbl_affine(const F& f, const Aff& q) {
 // Lazily reduced: P's coordinates below 2p (the identity has ZZ exactly 0, and a finite point's ZZ
 // is never 0 mod p), products without their final subtraction, R and Q - X3 as unreduced operands,
 // X3 from one signed combination; the result's coordinates are below 2p.
+// With neg the point added is -(x2, y2): the sign goes into R = -S2 - Y1 (+ 4p), and an identity
+// P becomes (x2, y2, 1, -1), the same point as (x2, -y2, 1, 1) since ZZ^3 = ZZZ^2 still holds,
+// so no coordinate is negated.
 template <class F>
-__device__ __forceinline__ Xyzz xyzz_madd(const F& f, const Xyzz& p, const Aff& q) {
+__device__ __forceinline__ Xyzz xyzz_madd(const F& f, const Xyzz& p, const Aff& q, bool neg = false) {
   if (fe_is_zero(p.zz)) {
     Xyzz r;
     r.x = q.x;
     r.y = q.y;
     r.zz = fe_const<F>(f.one);
-    r.zzz = r.zz;
+    r.zzz = neg ? fe_const<F>(f.mone) : r.zz;
     return r;
   }
   const Fe u2 = mont_raw(f, q.x, p.zz);
   const Fe s2 = mont_raw(f, q.y, p.zzz);
   const Fe pp0 = fsub_lazy(f, u2, p.x);  // [0, 2p)
-  if (fe_is_zero(pp0) || fe_eq(pp0, fe_const<F>(f.p)))
-    return fe_eq(canonical(f, s2), canonical(f, p.y)) ? xyzz_dbl_affine(f, q) : xyzz_identity(f);
-  const Fe rr = fsub_norm(f, s2, p.y);  // R + 2p, below 4p
+  if (fe_is_zero(pp0) || fe_eq(pp0, fe_const<F>(f.p))) {
+    Aff qs = q;
+    if (neg) qs.y = fneg(f, q.y);
+    return fe_eq(canonical(f, neg ? fneg(f, canonical(f, s2)) : s2), canonical(f, p.y)) ? xyzz_dbl_affine(f, qs)
+                                                                                        : xyzz_identity(f);
+  }
+  const Fe rr = fsub_norm_signed(f, s2, p.y, neg);  // +-S2 - Y1 + 2p or 4p, in (0, 4p]
   const Fe pp = sqr_raw(f, pp0);
   const Fe ppp = mont_raw(f, pp0, pp);
   const Fe qq = mont_raw(f, p.x, pp);

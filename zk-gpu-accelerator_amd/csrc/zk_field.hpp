@@ -376,6 +376,23 @@ __device__ __forceinline__ Fe fsub_norm(const F& f, const Fe& a, const Fe& b) {
   return d;
 }
 
+// (neg ? -a : a) - b + (neg ? 4p : 2p), in (0, 4p] with normalized limbs (signed carries), for
+// a, b < 2p with normalized limbs: each limb sum lies in (-2^30, 2^30 + 2^29 + 4), inside 32 bits.
+template <class F>
+__device__ __forceinline__ Fe fsub_norm_signed(const F& f, const Fe& a, const Fe& b, bool neg) {
+  const int32_t sgn = neg ? -1 : 0;  // (x ^ sgn) - sgn is -x when neg, else x
+  const int32_t k = neg ? 2 : 1;
+  Fe d;
+  int32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const int32_t t = (((int32_t)a.l[i] ^ sgn) - sgn) - (int32_t)b.l[i] + k * (int32_t)f.p2[i] + c;
+    d.l[i] = (uint32_t)t & LMASK;
+    c = t >> LB;
+  }
+  return d;
+}
+
 // x^2 * 2^-261 below 2p with the field's product schedule (the squaring shortcut for serial columns).
 template <class F>
 __device__ __forceinline__ Fe sqr_raw(const F& f, const Fe& a) {

@@ -805,7 +805,10 @@ template <class F>
 struct BucketSum<F, false> {
   Jac a;
   __device__ __forceinline__ void reset(const F& f) { a = jac_identity(f); }
-  __device__ __forceinline__ void add(const F& f, const Aff& q) { a = jac_madd(f, a, q); }
+  __device__ __forceinline__ void add(const F& f, Aff q, bool neg) {
+    if (neg) q.y = fneg(f, q.y);
+    a = jac_madd(f, a, q);
+  }
   __device__ __forceinline__ Jac jac(const F&) const { return a; }
   __device__ __forceinline__ void store(BucketRec* p) const { store_bucket(p, a); }
 };
@@ -814,7 +817,7 @@ template <class F>
 struct BucketSum<F, true> {
   Xyzz a;
   __device__ __forceinline__ void reset(const F& f) { a = xyzz_identity(f); }
-  __device__ __forceinline__ void add(const F& f, const Aff& q) { a = xyzz_madd(f, a, q); }
+  __device__ __forceinline__ void add(const F& f, const Aff& q, bool neg) { a = xyzz_madd(f, a, q, neg); }
   __device__ __forceinline__ Jac jac(const F& f) const { return xyzz_to_jac(f, a); }
   __device__ __forceinline__ void store(BucketRec* p) const { store_bucket(p, a); }
 };
@@ -857,9 +860,7 @@ k_accumulate(const uint2* __restrict__ sorted, const uint32_t* __restrict__ m_de
         cur = rec.x;
         acc.reset(f);
       }
-      Aff q = unpack_aff(pt);
-      if (rec.y >> 31) q.y = fneg(f, q.y);
-      acc.add(f, q);
+      acc.add(f, unpack_aff(pt), rec.y >> 31);
       rec = rec1;
       rec1 = rec2;
       pt = pt1;
