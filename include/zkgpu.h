@@ -206,8 +206,12 @@ int zk_msm_plan_create(void* srs, int count, const uint64_t* const* scalars, con
                        const int32_t* window, int chunk_len, uint64_t* out_jac, void** plan, void* stream);
 int zk_msm_plan_launch(void* plan, void* stream);
 int zk_msm_plan_destroy(void* plan);
-/* Device bytes zk_msm_plan_create holds for these sizes and windows (for admission by HBM budget). */
-int zk_msm_workspace_bytes(int count, const int64_t* n, const int32_t* window, int chunk_len, uint64_t* bytes);
+/* Device bytes zk_msm_plan_create holds for these sizes, windows and chunk_len over srs (for
+ * admission by HBM budget; the same window-table choice as the plan). srs may be NULL: plans of
+ * zk_msm_g1 over raw points. */
+int zk_msm_workspace_bytes(void* srs, int count, const int64_t* n, const int32_t* window, int chunk_len, uint64_t* bytes);
+/* Device bytes a created plan holds. */
+int zk_msm_plan_bytes(void* plan, uint64_t* bytes);
 /* Plan, launch and release in one call. */
 int zk_msm_g1_batch(void* srs, int count, const uint64_t* const* scalars, const int64_t* n,
                     const int32_t* window, uint64_t* out_jac, void* stream);

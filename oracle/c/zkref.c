@@ -304,3 +304,22 @@ int zr_fixed_base_g1(const uint64_t* k, uint64_t* out, long n, int threads) {
   for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
   return 0;
 }
+
+/* ---------------- Closed form of an MSM over a basis P_i = k_i G ---------------- */
+/* For P_i = k_i G, sum_i s_i P_i = (sum_i k_i s_i mod r) G (msm.py:112-207 returns that unique
+ * group element). out = sum_{i<n} k_i s_i mod r (plain), for plain k_i, s_i < 2^256 (reduced mod r
+ * first, as msm_gpu.py:433 reduces scalars). Montgomery products carry one factor 2^-256 per
+ * term, so the sum is brought back by one product with 2^512 mod r. */
+static fe reduce_r(const uint64_t* limbs) {
+  fe v = f_from(limbs);
+  while (geq(v.v, FR.p)) sub4(v.v, v.v, FR.p);
+  return v;
+}
+int zr_dot_mod_r(const uint64_t* k, const uint64_t* s, long n, uint64_t* out) {
+  ensure();
+  fe acc = {{0, 0, 0, 0}};
+  for (long i = 0; i < n; ++i) acc = f_add(&FR, acc, f_mul(&FR, reduce_r(k + 4 * i), reduce_r(s + 4 * i)));
+  acc = f_mul(&FR, acc, FR.r2);
+  memcpy(out, acc.v, 32);
+  return 0;
+}

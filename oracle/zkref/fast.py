@@ -21,19 +21,22 @@ _lib = None
 
 
 def lib() -> ctypes.CDLL:
+    """The loaded library, its signatures declared before it is shared (callers may be threads)."""
     global _lib
     if _lib is None:
         if not _LIB_PATH.exists():
             raise RuntimeError(f"{_LIB_PATH} is not built; run `make -C oracle`")
-        _lib = ctypes.CDLL(str(_LIB_PATH))
+        dll = ctypes.CDLL(str(_LIB_PATH))
         u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
         L = ctypes.c_long
-        _lib.zr_ntt.argtypes = [u64p, u64p, ctypes.c_int, u64p, ctypes.c_void_p]
-        _lib.zr_scale_by_powers.argtypes = [u64p, u64p, L, u64p]
-        _lib.zr_pointwise_quotient.argtypes = [u64p, u64p, u64p, u64p, u64p, L]
-        _lib.zr_g1_op.argtypes = [ctypes.c_int, u64p, L, u64p, L, u64p, L]
-        _lib.zr_msm_pippenger.argtypes = [u64p, u64p, L, ctypes.c_int, u64p, ctypes.POINTER(ctypes.c_int)]
-        _lib.zr_fixed_base_g1.argtypes = [u64p, u64p, L, ctypes.c_int]
+        dll.zr_ntt.argtypes = [u64p, u64p, ctypes.c_int, u64p, ctypes.c_void_p]
+        dll.zr_scale_by_powers.argtypes = [u64p, u64p, L, u64p]
+        dll.zr_pointwise_quotient.argtypes = [u64p, u64p, u64p, u64p, u64p, L]
+        dll.zr_g1_op.argtypes = [ctypes.c_int, u64p, L, u64p, L, u64p, L]
+        dll.zr_msm_pippenger.argtypes = [u64p, u64p, L, ctypes.c_int, u64p, ctypes.POINTER(ctypes.c_int)]
+        dll.zr_fixed_base_g1.argtypes = [u64p, u64p, L, ctypes.c_int]
+        dll.zr_dot_mod_r.argtypes = [u64p, u64p, L, u64p]
+        _lib = dll
     return _lib
 
 
@@ -108,3 +111,15 @@ def fixed_base_g1(ks: np.ndarray, threads: int | None = None) -> np.ndarray:
     t = threads or min(16, len(os.sched_getaffinity(0)))
     lib().zr_fixed_base_g1(ks, out, ks.shape[0], t)
     return out.reshape(-1, 2, 4)
+
+
+def dot_mod_r(keys: np.ndarray, scalars: np.ndarray) -> int:
+    """sum k_i s_i mod r over plain (n, 4) rows (any values below 2^256): for a basis P_i = k_i G
+    the MSM of the scalars s_i is this multiple of G."""
+    k = _limbs(keys).reshape(-1, 4)
+    s = _limbs(scalars).reshape(-1, 4)
+    if k.shape != s.shape:
+        raise ValueError("keys and scalars must have the same number of rows")
+    out = np.zeros(4, dtype=np.uint64)
+    lib().zr_dot_mod_r(k, s, k.shape[0], out)
+    return limbs_to_ints(out.reshape(1, 4))[0]

@@ -74,3 +74,21 @@ def test_fixed_base_and_msm():
         assert got == msm_pippenger([G * k for k in ks], sc, window_size=w).affine()
     assert got == (G * (sum(k * s for k, s in zip(ks, sc)) % R)).affine()
     assert fast.msm_pippenger(pts[:2], ints_to_limbs([0, 0]), 3) is None
+
+
+def test_dot_mod_r_is_the_msm_multiple():
+    """dot_mod_r is the scalar whose multiple of G equals the MSM over P_i = k_i G: pinned by the
+    definition on Python ints (operands at and above r included, as msm_gpu.py:433 reduces
+    scalars), by the oracle's Pippenger over those points, and by a planted swap of one pair."""
+    rng = random.Random(11)
+    ks = [rng.randrange(1, R) for _ in range(30)] + [R - 1, R, 2**256 - 1]
+    sc = [rng.randrange(R) for _ in range(30)] + [2**256 - 1, 5, R + 3]
+    got = fast.dot_mod_r(ints_to_limbs(ks), ints_to_limbs(sc))
+    assert got == sum(k * s for k, s in zip(ks, sc)) % R
+    pts = [G * (k % R) for k in ks[:30]]
+    assert (G * fast.dot_mod_r(ints_to_limbs(ks[:30]), ints_to_limbs(sc[:30]))).affine() == \
+        msm_pippenger(pts, sc[:30], window_size=5).affine()
+    swapped = sc[:]
+    swapped[0], swapped[1] = swapped[1], swapped[0]
+    assert fast.dot_mod_r(ints_to_limbs(ks), ints_to_limbs(swapped)) != got
+    assert fast.dot_mod_r(ints_to_limbs([]), ints_to_limbs([])) == 0

@@ -13,9 +13,9 @@ import numpy as np
 import pytest
 
 from zkgpu import gpu_backend as gb, workloads
-from zkgpu.batch import SRS, QuotientBatch, capture, msm_workspace_bytes, quotient_workspace_bytes
+from zkgpu.batch import SRS, MsmBatch, QuotientBatch, capture, msm_workspace_bytes, quotient_workspace_bytes, tuned_chunks
 from zkgpu.field256_gpu import BN254_FR
-from zkgpu.serve import ProofServer, WitnessPools, run_plain, served_workload
+from zkgpu.serve import GpuExecutor, ProofServer, WitnessPools, run_plain, served_workload
 from zkref import fast
 from zkref.curve import G
 from zkref.field import ints_to_limbs, limbs_to_ints
@@ -136,3 +136,22 @@ def test_served_over_window_tables_equal_alone():
     rep = ProofServer(srs, hbm_budget=1 << 31, slots=2, max_batch_jobs=4).serve(jobs)
     assert rep.batches >= 3
     check_against_alone(jobs, srs, pools)
+
+
+def test_footprint_covers_what_a_batch_allocates():
+    """Admission charges each job at least what it adds to any batch: over an SRS with and
+    without window tables, for every accumulation chunk a batch may use, the jobs' footprints
+    sum to no less than the batch's MSM plan, quotient workspace and h rows."""
+    keys = workloads.srs_keys(1 << 13, 0x5EED0C)
+    for tables in ([], [12]):
+        srs = SRS.from_keys(ints_to_limbs(keys)).precompute(tables)
+        ex = GpuExecutor(srs, slots=1)
+        pools = WitnessPools((1 << 13) + 5, seed=0x5EED0D)
+        for sizes in ([8, 13], [8, 8, 9, 13, 10], [11], [13, 13, 13]):
+            jobs = [pools.job(i, k, 0.0) for i, k in enumerate(sizes)]
+            charged = sum(ex.footprint(j) for j in jobs)
+            for chunk in tuned_chunks():
+                m = MsmBatch(srs, [s for j in jobs for s in j.scalars], windows=[0] * (3 * len(jobs)), chunk_len=chunk)
+                need = m.nbytes + quotient_workspace_bytes(sizes) + sum(j.n * 32 for j in jobs)
+                assert charged >= need, (tables, sizes, chunk, charged, need)
+                del m

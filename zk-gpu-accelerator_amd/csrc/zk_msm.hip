@@ -53,6 +53,8 @@
 // The result is a unique group element, so it equals the reference's
 // msm_pippenger for every input; intermediate coordinates never leave the device.
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/zkgpu.h"
@@ -690,6 +692,7 @@ struct alignas(16) RunSummary {
   JacRec fsum, lsum;
 };
 constexpr uint32_t kValid = 1, kSingle = 2;
+static_assert(sizeof(RunSummary) % 16 == 0 && sizeof(JacRec) % 16 == 0, "summaries are copied in 16-byte words");
 
 // Who performs a merge's addition: one thread, the 16 lanes of a row (wave-parallel formulas), or
 // a whole wavefront (limb-sliced formulas); the first lane of a row or wavefront does the writes.
@@ -911,14 +914,17 @@ __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in
     jl[k] = k > 0 && valid[k - 1] && valid[k] && ke[k - 1] == fk[k];
     jr[k] = k < 4 && valid[k] && valid[k + 1] && ke[k] == fk[k + 1];
   }
+  RunSummary* o = red + t;
   if (!valid[2]) {
-    red[t].flags = 0;
+    o->flags = 0;
     return;
   }
-  RunSummary r;
-  r.flags = 0;
   const FqIlp f;
   const RunSummary& me = in[t];
+  // What goes on to the trees: nothing, the whole summary, or one of its runs as a single run.
+  // (The output is written field by field from registers: a RunSummary assembled on the stack
+  // would live in scratch memory.)
+  enum { kNone, kWhole, kFirst, kLast } pass = kNone;
   // the one addition a thread may owe: the first fragment of summary t (its single run or its
   // first run) as the right end of a chain of two, added to the fragment that ends summary t - 1;
   // every case below reaches the same call, so lanes that take different cases do not each pay
@@ -929,9 +935,7 @@ __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in
     if (!lo && !hi) {
       store_bucket(buckets + fk[2], load_jac_rec(&me.fsum));
     } else if ((lo && hi) || (lo && single[1] && jl[1]) || (hi && single[3] && jr[3])) {
-      r.fk = r.lk = fk[2];
-      r.flags = kValid | kSingle;
-      r.fsum = me.fsum;
+      pass = kFirst;
     } else {
       add = lo;  // else: a chain of two that the next summary's thread adds up
     }
@@ -942,20 +946,27 @@ __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in
     else add = true;
     if (!jr[2]) store_bucket(buckets + me.lk, load_jac_rec(&me.lsum));
     else if (single[3] && jr[3]) last_in_tree = true;
-    if (first_in_tree && last_in_tree) {
-      r = me;
-    } else if (first_in_tree || last_in_tree) {
-      r.fk = r.lk = first_in_tree ? me.fk : me.lk;
-      r.flags = kValid | kSingle;
-      r.fsum = first_in_tree ? me.fsum : me.lsum;
-    }
+    pass = first_in_tree && last_in_tree ? kWhole : first_in_tree ? kFirst : last_in_tree ? kLast : kNone;
   }
   if (add) {
     const JacRec* left = single[1] ? &in[t - 1].fsum : &in[t - 1].lsum;
     store_bucket(buckets + fk[2], jac_add_lazy(f, load_jac_rec(left), load_jac_rec(&me.fsum)));
   }
-  if (r.flags) red[t] = r;
-  else red[t].flags = 0;
+  if (pass == kNone) {
+    o->flags = 0;
+  } else if (pass == kWhole) {
+    const uint4* src = reinterpret_cast<const uint4*>(&me);
+    uint4* dst = reinterpret_cast<uint4*>(o);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(RunSummary) / 16); ++i) dst[i] = src[i];
+  } else {
+    const uint32_t key = pass == kFirst ? fk[2] : me.lk;
+    *reinterpret_cast<uint4*>(o) = make_uint4(key, key, kValid | kSingle, 0u);
+    const uint4* src = reinterpret_cast<const uint4*>(pass == kFirst ? &me.fsum : &me.lsum);
+    uint4* dst = reinterpret_cast<uint4*>(&o->fsum);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(JacRec) / 16); ++i) dst[i] = src[i];
+  }
 }
 
 // The `count` summaries of the previous level, 256 per workgroup, merged by a tree (empty
@@ -1408,21 +1419,11 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   return 0;
 }
 
-int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, hipStream_t stream, MsmPlan** out) {
-  auto plan = new MsmPlan();
-  plan->points = points;
-  MsmLayout L;
-  if (int rc = msm_plan_geometry(plan, jobs, chunk, &L)) {
-    delete plan;
-    return rc;
-  }
-  plan->mem = dev_alloc(L.total, stream);
-  if (!plan->mem) {
-    delete plan;
-    return 1;
-  }
+// Points every buffer of the plan at its offset in the block at base (the plan's memory).
+static void msm_plan_bind(MsmPlan* plan, const MsmLayout& L, void* mem) {
+  plan->mem = mem;
   plan->bytes = L.total;
-  char* base = static_cast<char*>(plan->mem);
+  char* base = static_cast<char*>(mem);
   plan->d_jobs = reinterpret_cast<MsmJob*>(base + L.jobs);
   plan->counts = reinterpret_cast<uint32_t*>(base + L.counts);
   plan->totals = reinterpret_cast<uint32_t*>(base + L.totals);
@@ -1444,6 +1445,22 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
   plan->parts_buf = reinterpret_cast<JacRec*>(base + L.parts);
   plan->wparts = reinterpret_cast<JacRec*>(base + L.wparts);
   plan->wsum = reinterpret_cast<JacRec*>(base + L.wsum);
+}
+
+int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, hipStream_t stream, MsmPlan** out) {
+  auto plan = new MsmPlan();
+  plan->points = points;
+  MsmLayout L;
+  if (int rc = msm_plan_geometry(plan, jobs, chunk, &L)) {
+    delete plan;
+    return rc;
+  }
+  void* mem = dev_alloc(L.total, stream);
+  if (!mem) {
+    delete plan;
+    return 1;
+  }
+  msm_plan_bind(plan, L, mem);
   // Stream-ordered upload from the plan's own job table, which lives as long as the plan.
   if (!plan->jobs.empty()) {
     hipError_t e = hipMemcpyAsync(plan->d_jobs, plan->jobs.data(), sizeof(MsmJob) * plan->jobs.size(),
@@ -1472,9 +1489,50 @@ static int msm_plan_clear(MsmPlan* plan, hipStream_t stream) {
   return 0;
 }
 
-// Enqueues the whole batch; allocates nothing.
+// MSM launches run one at a time on the device, whatever stream issues them: each launch waits
+// for the previous one's completion event and records its own. Two MSM launch sequences executing
+// side by side on two streams gave wrong (still on-curve) results for about 1 MSM in 10^4, while an
+// MSM beside NTT or copy kernels on another stream, and MSMs on two streams one after the other,
+// gave none in several 10^5; the cause was not found, so MSMs are kept from overlapping each other.
+// Work on other streams (quotients, NTTs) still overlaps them. Launches recorded into a HIP graph
+// take no part in this order (a replay must not run beside other MSM work).
+static std::mutex g_order_mu;
+static hipEvent_t g_order_ev = nullptr;
+
+static int msm_order_enter(hipStream_t stream, bool* ordered) {
+  *ordered = false;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  ZK_HIP(hipStreamIsCapturing(stream, &cap));
+  if (cap != hipStreamCaptureStatusNone) return 0;
+  std::lock_guard<std::mutex> lk(g_order_mu);
+  if (!g_order_ev) ZK_HIP(hipEventCreateWithFlags(&g_order_ev, hipEventDisableTiming));
+  ZK_HIP(hipStreamWaitEvent(stream, g_order_ev, 0));
+  *ordered = true;
+  return 0;
+}
+
+static int msm_order_leave(hipStream_t stream) {
+  std::lock_guard<std::mutex> lk(g_order_mu);
+  ZK_HIP(hipEventRecord(g_order_ev, stream));
+  return 0;
+}
+
+static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream);
+
+// Enqueues the whole batch after every earlier MSM launch; allocates nothing.
 int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
   if (plan->njobs == 0) return 0;
+  bool ordered = false;
+  if (int rc = msm_order_enter(stream, &ordered)) return rc;
+  const int rc = msm_plan_launch_kernels(plan, stream);
+  if (ordered) {
+    // recorded even after a failed launch, so that later launches do not wait on a stale order
+    if (int rc2 = msm_order_leave(stream)) return rc ? rc : rc2;
+  }
+  return rc;
+}
+
+static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream) {
   const int nj = plan->njobs;
   if (int rc = msm_plan_clear(plan, stream)) return rc;
   {
@@ -1730,20 +1788,18 @@ static const SrsTable* pick_table(const ZkSrs* s, int64_t n, int window) {
   return best;
 }
 
-int zk_msm_plan_create(void* srs, int count, const uint64_t* const* scalars, const int64_t* n, const int32_t* window,
-                       int chunk_len, uint64_t* out_jac, void** plan, void* stream) {
-  auto* s = static_cast<ZkSrs*>(srs);
-  if (!s) return fail("an SRS handle is required");
-  std::vector<MsmJob> jobs((size_t)std::max(count, 0));
+// The job table of count MSMs over prefixes of an SRS (window tables picked as plans pick them).
+static int srs_jobs(const ZkSrs* s, int count, const uint64_t* const* scalars, const int64_t* n, const int32_t* window,
+                    uint64_t* out_jac, std::vector<MsmJob>& jobs) {
+  jobs.assign((size_t)std::max(count, 0), MsmJob{});
   for (int j = 0; j < count; ++j) {
     if (n[j] < 0 || n[j] > s->n) return fail("MSM size exceeds the SRS");
     MsmJob& job = jobs[(size_t)j];
-    job = MsmJob{};
-    job.scalars = scalars[j];
+    job.scalars = scalars ? scalars[j] : nullptr;
     job.phi_base = s->n;
     job.n = n[j];
     job.c = window ? window[j] : 0;
-    job.out = out_jac + 12 * j;
+    job.out = out_jac ? out_jac + 12 * j : nullptr;
     if (const SrsTable* t = pick_table(s, n[j], job.c)) {
       job.table = 1;
       job.c = t->c;
@@ -1751,6 +1807,15 @@ int zk_msm_plan_create(void* srs, int count, const uint64_t* const* scalars, con
       job.table_stride = s->n;
     }
   }
+  return 0;
+}
+
+int zk_msm_plan_create(void* srs, int count, const uint64_t* const* scalars, const int64_t* n, const int32_t* window,
+                       int chunk_len, uint64_t* out_jac, void** plan, void* stream) {
+  auto* s = static_cast<ZkSrs*>(srs);
+  if (!s) return fail("an SRS handle is required");
+  std::vector<MsmJob> jobs;
+  if (int rc = srs_jobs(s, count, scalars, n, window, out_jac, jobs)) return rc;
   MsmPlan* p = nullptr;
   if (int rc = msm_plan_create(s->pts, jobs, chunk_len, (hipStream_t)stream, &p)) return rc;
   p->srs = s;
@@ -1759,17 +1824,28 @@ int zk_msm_plan_create(void* srs, int count, const uint64_t* const* scalars, con
   return 0;
 }
 
-int zk_msm_workspace_bytes(int count, const int64_t* n, const int32_t* window, int chunk_len, uint64_t* bytes) {
-  std::vector<MsmJob> jobs((size_t)std::max(count, 0));
-  for (int j = 0; j < count; ++j) {
-    jobs[(size_t)j] = MsmJob{};
-    jobs[(size_t)j].n = n[j];
-    jobs[(size_t)j].c = window ? window[j] : 0;
+int zk_msm_workspace_bytes(void* srs, int count, const int64_t* n, const int32_t* window, int chunk_len, uint64_t* bytes) {
+  std::vector<MsmJob> jobs;
+  if (srs) {
+    if (int rc = srs_jobs(static_cast<ZkSrs*>(srs), count, nullptr, n, window, nullptr, jobs)) return rc;
+  } else {
+    jobs.assign((size_t)std::max(count, 0), MsmJob{});
+    for (int j = 0; j < count; ++j) {
+      jobs[(size_t)j].n = n[j];
+      jobs[(size_t)j].c = window ? window[j] : 0;
+    }
   }
   MsmPlan plan;
   MsmLayout L;
   if (int rc = msm_plan_geometry(&plan, jobs, chunk_len, &L)) return rc;
   *bytes = L.total;
+  return 0;
+}
+
+int zk_msm_plan_bytes(void* plan, uint64_t* bytes) {
+  auto* p = static_cast<MsmPlan*>(plan);
+  if (!p) return fail("a plan handle is required");
+  *bytes = p->bytes;
   return 0;
 }
 

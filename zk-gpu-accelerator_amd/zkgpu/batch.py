@@ -121,6 +121,13 @@ class MsmBatch:
     def launch(self) -> None:
         _gb.call("zk_msm_plan_launch", self.handle, _gb.current_stream())
 
+    @property
+    def nbytes(self) -> int:
+        """Device bytes the plan holds."""
+        out = ctypes.c_uint64()
+        _gb.call("zk_msm_plan_bytes", self.handle, ctypes.byref(out))
+        return int(out.value)
+
     def results(self) -> list[Point]:
         rows = BN254_FQ.to_limbs(self.out.reshape(-1, 4)).get()
         flat = _limbs_to_ints(rows)
@@ -218,13 +225,29 @@ def quotient_workspace_bytes(log_n: Sequence[int]) -> int:
     return int(out.value)
 
 
-def msm_workspace_bytes(sizes: Sequence[int], windows: Sequence[int] | None = None) -> int:
-    """Device bytes an MsmBatch over these sizes holds (besides its scalars and results)."""
+def msm_workspace_bytes(sizes: Sequence[int], windows: Sequence[int] | None = None, *, srs: SRS | None = None,
+                        chunk_len: int | None = None) -> int:
+    """Device bytes an MsmBatch over these sizes holds (besides its scalars and results): over
+    `srs` the plan's own window-table choice applies; chunk_len defaults to the tuned chunk of the
+    largest size, as MsmBatch picks it."""
     out = ctypes.c_uint64()
     count = len(sizes)
-    wins = (ctypes.c_int32 * max(count, 1))(*(windows if windows is not None else [_tuned_window(n) for n in sizes]))
-    _gb.call("zk_msm_workspace_bytes", count, (ctypes.c_int64 * max(count, 1))(*sizes), wins, 0, ctypes.byref(out))
+    if windows is None:
+        windows = [0] * count if srs is not None and getattr(srs, "tables", None) else [_tuned_window(n) for n in sizes]
+    if chunk_len is None:
+        chunk_len = _tuned_chunk(sizes)
+    wins = (ctypes.c_int32 * max(count, 1))(*windows)
+    _gb.call("zk_msm_workspace_bytes", srs.handle if srs is not None else None, count,
+             (ctypes.c_int64 * max(count, 1))(*sizes), wins, int(chunk_len), ctypes.byref(out))
     return int(out.value)
+
+
+def tuned_chunks() -> list[int]:
+    """Every accumulation chunk length the tuning table of this GPU may pick (0 = persistent grid)."""
+    from zkgpu.autotune import load
+
+    table = load() or {}
+    return sorted({0} | {int(v) for v in table.get("msm_chunk", {}).values()})
 
 
 def capture(work: Sequence[Any], on: _gb.Stream) -> _gb.Graph:

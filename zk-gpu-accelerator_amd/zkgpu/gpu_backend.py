@@ -121,7 +121,8 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_msm_plan_create": [_vp, _i32, _vp, _vp, _vp, _i32, _vp, ctypes.POINTER(_vp), _vp],
     "zk_msm_plan_launch": [_vp, _vp],
     "zk_msm_plan_destroy": [_vp],
-    "zk_msm_workspace_bytes": [_i32, _vp, _vp, _i32, ctypes.POINTER(_u64)],
+    "zk_msm_workspace_bytes": [_vp, _i32, _vp, _vp, _i32, ctypes.POINTER(_u64)],
+    "zk_msm_plan_bytes": [_vp, ctypes.POINTER(_u64)],
     "zk_msm_g1_batch": [_vp, _i32, _vp, _vp, _vp, _vp, _vp],
     "zk_ntt_batch": [_i32, _vp, _vp, _vp, _vp, _vp],
     "zk_ntt_plan_create": [_i32, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp],

@@ -245,13 +245,18 @@ class GpuExecutor:
         return self.windows(n) if self.windows else 0
 
     def footprint(self, job: ProofJob) -> int:
-        from zkgpu.batch import msm_workspace_bytes, quotient_workspace_bytes
+        """Device bytes the job adds to a batch: its h rows, its quotient workspace and its three
+        MSMs' plan buffers, the latter sized over the SRS (whose window tables change the entry and
+        bucket counts) for the accumulation chunk that needs the most memory among those a batch
+        may use (the chunk is the tuned one of the batch's largest MSM)."""
+        from zkgpu.batch import msm_workspace_bytes, quotient_workspace_bytes, tuned_chunks
 
         fp = self._footprints.get(job.log_n)
         if fp is None:
             n = job.n
-            fp = n * 32 + quotient_workspace_bytes([job.log_n]) + msm_workspace_bytes([n] * 3, [self._window(n)] * 3)
-            fp += 3 * 96 + 4096
+            msm = max(msm_workspace_bytes([n] * 3, [self._window(n)] * 3, srs=self.srs, chunk_len=ch)
+                      for ch in tuned_chunks())
+            fp = n * 32 + quotient_workspace_bytes([job.log_n]) + msm + 3 * 96 + 4096
             self._footprints[job.log_n] = fp
         return fp
 

@@ -11,6 +11,10 @@ Seeded synthetic inputs for the benchmark configurations (no datasets exist for 
   served workload).
 - log_uniform_sizes: sizes 2^U with U uniform in [lo, hi] for the batched and
   served workloads.
+- batched_workload / batched_msm_scalars / batched_ntt_rows: the batched form's
+  jobs (64 MSMs with sizes log-uniform in [2^10, 2^20] over prefixes of the
+  2^20-point basis with seed 0x5EED04, witness-like scalars with a seed per
+  MSM; 32 forward NTTs of 2^k rows, k uniform in {10..20}).
 All generators return plain little-endian (n, 4) uint64 limbs unless noted.
 """
 
@@ -119,3 +123,25 @@ def log_uniform_pow2(count: int, lo: int, hi: int, seed: int) -> list[int]:
 def closed_form_scalar(keys: list[int], scalars: list[int]) -> int:
     """sum s_i k_i mod r: for P_i = k_i G the MSM equals this multiple of G."""
     return sum(k * s for k, s in zip(keys, scalars)) % R
+
+
+# ---- the batched form ---------------------------------------------------------------------
+
+BATCHED_SRS_LOG = 20
+BATCHED_SRS_SEED = 0x5EED04
+
+
+def batched_workload() -> tuple[list[int], list[int]]:
+    """(MSM sizes, NTT sizes): 64 MSMs floor(2^U), U uniform in [10, 20], and 32 NTTs of 2^k
+    rows, k uniform in {10..20}."""
+    return log_uniform_sizes(64, 10, 20, 0x5EED10), log_uniform_pow2(32, 10, 20, 0x5EED11)
+
+
+def batched_msm_scalars(j: int, n: int) -> np.ndarray:
+    """Witness-like plain scalar rows of MSM j of the batched form."""
+    return witness_rows_fast(n, 0x5EED1000 + j)
+
+
+def batched_ntt_rows(i: int, n: int) -> np.ndarray:
+    """Uniform plain Fr rows, the input of NTT i of the batched form."""
+    return uniform_rows(n, 0x5EED2000 + i)
