@@ -164,6 +164,17 @@ int zk_g1_fixed_base(const uint64_t* k, int64_t n, uint64_t* out_affine, void* s
  * (0 = automatic); chunk_len: bucket entries per thread in the accumulation (0 = automatic). */
 int zk_msm_g1(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window,
               int chunk_len, uint64_t* out_jac, void* stream);
+/* zk_msm_g1 with no allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, kernels and Jacobian result as zk_msm_g1; every device buffer of the
+ * call (point records, digits, sort, buckets, run summaries, reduction) lives in one file-scope
+ * device array of zk_msm_noalloc_max_bytes bytes (64 MiB) that each GPU holds until the process
+ * ends, so only MSMs whose workspace fits are accepted (2^14 points with the automatic window and
+ * chunk; larger ones fail with an error naming the bytes needed). Calls on one GPU share that array
+ * and keep the last call's contents: calls that overlap on one GPU (two streams, two threads) race. */
+int zk_msm_g1_noalloc(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window,
+                      int chunk_len, uint64_t* out_jac, void* stream);
+int zk_msm_noalloc_max_bytes(uint64_t* bytes);
 /* The signed-digit window zk_msm_g1 picks for n points when window = 0. */
 int zk_msm_default_window(int64_t n, int* window);
 /* Whether plans created afterwards split each scalar k = k1 + k2 lambda (|k1|, |k2| < 2^127)

@@ -1653,6 +1653,81 @@ int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, in
   return rc;
 }
 
+// ---- the MSM entry that allocates nothing (zk_msm_g1_noalloc) ------------------------------
+
+// Device memory of zk_msm_g1_noalloc: point records first, then the plan's buffers, laid out as
+// zk_msm_g1 lays out its workspace. Each GPU holds its own copy of this array from the module's
+// load until the process ends. On the host the name only stands for the symbol: host code uses
+// the address hipGetSymbolAddress gives for the current device, never the name itself.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kMaxDevices];  // per device: the device address of scratch
+
+// Looks up the current device's copy of scratch once per device, with that device current
+// (zk_init calls it; zk_msm_g1_noalloc calls it again on a device it has not seen).
+int msm_scratch_init() {
+  int dev = 0;
+  ZK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("device index out of range for the MSM scratch");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* addr = nullptr;
+    ZK_HIP(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+    if (!addr) return fail("no device address for the MSM scratch");
+    g_scratch_addr[dev] = addr;
+  }
+  return 0;
+}
+
+static void* scratch_addr() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_addr[dev];
+}
+
+// Writes the job table from a kernel argument (no host-to-device copy).
+__global__ void k_store_job(MsmJob job, MsmJob* dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = job;
+}
+
+// zk_msm_g1 with every device buffer in scratch: the same plan geometry, kernels and launch order
+// for the same arguments, so the same Jacobian result.
+int msm_g1_noalloc(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, int k0, uint64_t* out_jac,
+                   hipStream_t stream) {
+  if (n < 0 || n > 0x3fffffff) return fail("MSM size must be in [0, 2^30)");
+  if (c < 0 || c > 20) return fail("window must be in 1..20");
+  if (int rc = msm_scratch_init()) return rc;
+  char* base = static_cast<char*>(scratch_addr());
+  if (!base) return fail("the MSM scratch has no device address");
+  const bool glv = g_glv != 0;
+  const size_t rec_bytes = align_up(sizeof(AffRec) * (size_t)std::max<int64_t>(glv ? 2 * n : n, 1));
+  MsmJob job{};
+  job.scalars = scalars;
+  job.n = n;
+  job.c = c;
+  job.out = out_jac;
+  job.phi_base = n;
+  MsmPlan plan;
+  MsmLayout L;
+  std::vector<MsmJob> jobs{job};
+  if (int rc = msm_plan_geometry(&plan, jobs, k0, &L)) return rc;
+  if (rec_bytes + L.total > kNoAllocScratchBytes)
+    return fail("zk_msm_g1_noalloc: an MSM of " + std::to_string(n) + " points with these parameters needs " +
+                std::to_string(rec_bytes + L.total) + " bytes of scratch, more than its " +
+                std::to_string(kNoAllocScratchBytes) + " (use zk_msm_g1)");
+  AffRec* pts = reinterpret_cast<AffRec*>(base);
+  if (int rc = convert_points(points, n, pts, glv, stream)) return rc;
+  plan.points = pts;
+  msm_plan_bind(&plan, L, base + rec_bytes);
+  hipLaunchKernelGGL(k_store_job, dim3(1), dim3(64), 0, stream, plan.jobs[0], plan.d_jobs);
+  ZK_LAUNCH_CHECK("msm job table");
+  return msm_plan_launch(&plan, stream);
+}
+
 }  // namespace zk
 
 using namespace zk;
@@ -1699,6 +1774,16 @@ int zk_msm_get_segment(int* log_seg) {
 int zk_msm_g1(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window, int chunk_len,
               uint64_t* out_jac, void* stream) {
   return msm_g1(points_affine, scalars, n, window, chunk_len, out_jac, (hipStream_t)stream);
+}
+
+int zk_msm_g1_noalloc(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window, int chunk_len,
+                      uint64_t* out_jac, void* stream) {
+  return msm_g1_noalloc(points_affine, scalars, n, window, chunk_len, out_jac, (hipStream_t)stream);
+}
+
+int zk_msm_noalloc_max_bytes(uint64_t* bytes) {
+  *bytes = kNoAllocScratchBytes;
+  return 0;
 }
 
 // An SRS on the device: records [0, n) hold P_i, [n, 2n) phi(P_i), and each window table of

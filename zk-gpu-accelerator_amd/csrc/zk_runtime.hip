@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/zkgpu.h"
+#include "zk_msm.hpp"
 
 namespace zk {
 
@@ -204,7 +205,7 @@ int zk_init(int device) {
   hipDeviceProp_t prop;
   ZK_HIP(hipGetDeviceProperties(&prop, device));
   g_cus = prop.multiProcessorCount;
-  return 0;
+  return msm_scratch_init();
 }
 
 int zk_device_info(char* name, int name_len, int* compute_units, int* clock_khz) {
