@@ -106,7 +106,12 @@ int zk_field_sub(int field_id, const uint64_t* a, int a_bcast, const uint64_t* b
  * (field, log_n, omega, coset). */
 int zk_ntt(int field_id, const uint64_t* in, uint64_t* out, int log_n, int inverse,
            const uint64_t omega[4], const uint64_t* coset, void* stream);
+/* Frees the cached tables that no plan holds (plans, and graphs recorded from them, keep theirs
+ * until destroyed); the cache also evicts such tables, least recently used first, beyond 4 GiB.
+ * Replaces _ntt_tables.cache_clear() (field256_gpu.py:492). zk_ntt_cache_stats reports the
+ * cached tables, their device bytes and how many are held by plans. */
 int zk_ntt_cache_clear(void);
+int zk_ntt_cache_stats(uint64_t* tables, uint64_t* bytes, uint64_t* pinned);
 /* Radix-2 stages fused per LDS-staged pass for transforms of size 2^log_n (1..10; 0 restores
  * the default of 8), as picked by the autotuner; plans created afterwards use it. No counterpart
  * in the reference, whose NTT runs one kernel launch per stage (src/zkgpu/ntt.py:149-203). */

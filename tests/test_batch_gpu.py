@@ -138,3 +138,53 @@ def test_window_table_needs_no_live_plan(tabled):
     import gc
 
     gc.collect()
+
+
+def cache_stats():
+    import ctypes
+
+    t, b, p = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    gb.call("zk_ntt_cache_stats", ctypes.byref(t), ctypes.byref(b), ctypes.byref(p))
+    return t.value, b.value, p.value
+
+
+def test_cache_clear_keeps_the_tables_of_live_plans():
+    """Clearing the table cache (the reference's _ntt_tables.cache_clear) frees only tables no plan
+    holds: a plan, and a graph recorded from it, still find theirs after a clear."""
+    import gc
+
+    from zkgpu.batch import QuotientBatch
+
+    gb.call("zk_ntt_cache_clear")
+    sizes = [1 << 12, 1 << 13]
+    vals, rows = ntt_inputs(sizes, 31)
+    outs = [gb.empty((n, 4)) for n in sizes]
+    s = gb.Stream()
+    with gb.stream(s.handle):
+        plan = NttBatch(rows, outs, [False, True])
+        qrows = [BN254_FR.from_limbs(workloads.uniform_rows(1 << 11, 90 + k)) for k in range(3)]
+        qb = QuotientBatch([qrows[0]], [qrows[1]], [qrows[2]])
+    s.synchronize()
+    graph = capture([plan, qb], s)
+    tables, _, pinned = cache_stats()
+    assert pinned >= 7 and tables >= pinned
+    gb.call("zk_ntt_cache_clear")
+    assert cache_stats()[0] == cache_stats()[2] == pinned
+    want = [fast.ntt_rows(ints_to_limbs(vals[0])), fast.ntt_rows(ints_to_limbs(vals[1]), invert=True)]
+    plain = [BN254_FR.to_limbs(x).get() for x in qrows]
+    want_h = fast.groth16_quotient(*plain, 7)
+    for rep in range(2):
+        with gb.stream(s.handle):
+            graph.replay()
+        s.synchronize()
+        for o, w in zip(outs, want):
+            assert np.array_equal(BN254_FR.to_limbs(o).get(), w), rep
+        assert np.array_equal(BN254_FR.to_limbs(qb.h[0]).get(), want_h), rep
+    del graph, plan, qb
+    gc.collect()
+    assert cache_stats()[2] == 0
+    gb.call("zk_ntt_cache_clear")
+    assert cache_stats()[:2] == (0, 0)
+    # a one-shot transform after the clear builds its tables again
+    got = BN254_FR.to_limbs(BN254_FR.ntt(rows[0])).get()
+    assert np.array_equal(got, want[0])

@@ -1269,6 +1269,8 @@ int default_window(int64_t n) {
 // allocates nothing (and the launch sequence can be captured into a graph).
 struct MsmPlan {
   int njobs = 0;
+  hipStream_t alloc_stream = nullptr;  // the stream whose allocator cache the memory returns to
+  bool escaped = false;                // launched on another stream or recorded into a graph
   std::vector<MsmJob> jobs;
   std::vector<int64_t> level_max;
   int64_t blocks = 0, nkeys = 0, max_entries = 0, max_slots = 0, max_slots1 = 0, segments = 0, parts = 0,
@@ -1460,6 +1462,7 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
     delete plan;
     return 1;
   }
+  plan->alloc_stream = stream;
   msm_plan_bind(plan, L, mem);
   // Stream-ordered upload from the plan's own job table, which lives as long as the plan.
   if (!plan->jobs.empty()) {
@@ -1477,6 +1480,7 @@ int msm_plan_create(const AffRec* points, std::vector<MsmJob> jobs, int chunk, h
 
 void msm_plan_destroy(MsmPlan* plan) {
   if (!plan) return;
+  if (plan->escaped) (void)hipDeviceSynchronize();  // work the allocating stream does not order
   dev_free(plan->mem);
   delete plan;
 }
@@ -1522,6 +1526,7 @@ static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream);
 // Enqueues the whole batch after every earlier MSM launch; allocates nothing.
 int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
   if (plan->njobs == 0) return 0;
+  if (plan->mem && launch_escapes(plan->alloc_stream, stream)) plan->escaped = true;
   bool ordered = false;
   if (int rc = msm_order_enter(stream, &ordered)) return rc;
   const int rc = msm_plan_launch_kernels(plan, stream);

@@ -292,11 +292,38 @@ struct TableKey {
 struct Table {
   uint64_t* ptr;
   size_t rows;
+  size_t bytes;
+  int uses;  // pins held by plans and by calls whose kernels are not yet enqueued
 };
 
 std::mutex g_tab_mu;
 std::map<TableKey, Table> g_tables;
-std::list<TableKey> g_lru;  // most recently used first (tables are never evicted: plans keep raw pointers)
+std::list<TableKey> g_lru;  // most recently used first
+size_t g_table_bytes = 0;
+// Device bytes the cache keeps before it evicts unpinned tables, least recently used first.
+constexpr size_t kTableCacheBytes = (size_t)4 << 30;
+thread_local TableHold* g_hold = nullptr;
+
+// Frees unpinned tables, least recently used first, until the cache holds at most `limit` bytes
+// (0: every unpinned table). Waits for the device first: kernels already enqueued may still read
+// any table. Called with g_tab_mu held.
+int evict_locked(size_t limit) {
+  bool synced = false;
+  for (auto it = g_lru.end(); it != g_lru.begin() && g_table_bytes > limit;) {
+    --it;
+    auto t = g_tables.find(*it);
+    if (t == g_tables.end() || t->second.uses > 0) continue;
+    if (!synced) {
+      ZK_HIP(hipDeviceSynchronize());
+      synced = true;
+    }
+    (void)hipFree(t->second.ptr);
+    g_table_bytes -= t->second.bytes;
+    g_tables.erase(t);
+    it = g_lru.erase(it);
+  }
+  return 0;
+}
 
 bool field_modulus(int field_id, U256* p) {
   static const uint64_t r[4] = {0x43e1f593f0000001ULL, 0x2833e84879b97091ULL, 0xb85045b68181585dULL, 0x30644e72e131a029ULL};
@@ -352,6 +379,10 @@ int get_table(const TableKey& key, hipStream_t stream, const uint64_t** out) {
   if (it != g_tables.end()) {
     g_lru.remove(key);
     g_lru.push_front(key);
+    if (g_hold) {
+      ++it->second.uses;
+      g_hold->pin(it->second.ptr);
+    }
     *out = it->second.ptr;
     return 0;
   }
@@ -361,7 +392,9 @@ int get_table(const TableKey& key, hipStream_t stream, const uint64_t** out) {
   const int64_t n = 1LL << key.log_n;
   size_t rows = key.kind == 0 ? (size_t)(n > 1 ? n - 1 : 1) : key.kind >= 3 ? 1 : (size_t)n;
   uint64_t* buf = nullptr;
-  ZK_HIP(hipMalloc(&buf, rows * (key.kind == 0 ? 8 * kTwWords : 32)));
+  const size_t bytes = rows * (key.kind == 0 ? 8 * kTwWords : 32);
+  if (int rc = evict_locked(kTableCacheBytes > bytes ? kTableCacheBytes - bytes : 0)) return rc;
+  ZK_HIP(hipMalloc(&buf, bytes));
   int rc = 0;
   if (key.kind == 0) {
     U256 omega = U256::from(key.a);
@@ -389,10 +422,22 @@ int get_table(const TableKey& key, hipStream_t stream, const uint64_t** out) {
     (void)hipFree(buf);
     return rc;
   }
-  g_tables[key] = Table{buf, rows};
+  g_tables[key] = Table{buf, rows, bytes, g_hold ? 1 : 0};
+  g_table_bytes += bytes;
   g_lru.push_front(key);
+  if (g_hold) g_hold->pin(buf);
   *out = buf;
   return 0;
+}
+
+// Drops one pin per listed table (g_tab_mu held).
+void unpin_locked(const std::vector<const uint64_t*>& pins) {
+  for (const uint64_t* p : pins)
+    for (auto& kv : g_tables)
+      if (kv.second.ptr == p) {
+        if (kv.second.uses > 0) --kv.second.uses;
+        break;
+      }
 }
 
 struct PassPlan {
@@ -425,12 +470,37 @@ std::vector<PassPlan> plan_passes(int log_n) {
 
 }  // namespace
 
+TableHold::TableHold() : prev_(g_hold) { g_hold = this; }
+
+TableHold::~TableHold() {
+  g_hold = prev_;
+  if (!pins_.empty()) tables_release(pins_);
+}
+
+std::vector<const uint64_t*> TableHold::take() {
+  std::vector<const uint64_t*> out;
+  out.swap(pins_);
+  return out;
+}
+
+void tables_release(const std::vector<const uint64_t*>& pins) {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  unpin_locked(pins);
+}
+
+// Frees every table no plan holds (pinned ones stay cached and valid).
 int ntt_cache_clear() {
   std::lock_guard<std::mutex> lk(g_tab_mu);
-  ZK_HIP(hipDeviceSynchronize());
-  for (auto& kv : g_tables) (void)hipFree(kv.second.ptr);
-  g_tables.clear();
-  g_lru.clear();
+  return evict_locked(0);
+}
+
+int ntt_cache_stats(uint64_t* tables, uint64_t* bytes, uint64_t* pinned) {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  uint64_t p = 0;
+  for (const auto& kv : g_tables) p += kv.second.uses > 0 ? 1 : 0;
+  *tables = g_tables.size();
+  *bytes = g_table_bytes;
+  *pinned = p;
   return 0;
 }
 
@@ -530,6 +600,7 @@ int batch_plan_create(const std::vector<TransformSpec>& specs, hipStream_t strea
     delete plan;
     return 1;
   }
+  plan->alloc_stream = stream;
   uint64_t* tmp = static_cast<uint64_t*>(plan->mem);
   NttPassDesc* dtab = reinterpret_cast<NttPassDesc*>(static_cast<char*>(plan->mem) + tmp_bytes);
   for (size_t j = 0; j < specs.size(); ++j) {
@@ -577,7 +648,8 @@ int batch_plan_create(const std::vector<TransformSpec>& specs, hipStream_t strea
   return 0;
 }
 
-int batch_plan_launch(const BatchPlan* plan, hipStream_t stream) {
+int batch_plan_launch(BatchPlan* plan, hipStream_t stream) {
+  if (launch_escapes(plan->alloc_stream, stream)) plan->escaped = true;
   const FrConst f;
   for (size_t i = 0; i < plan->counts.size(); ++i) {
     launch_timed("ntt_pass_multi", k_ntt_pass_multi<FrConst>, dim3((unsigned)plan->tiles[i]), dim3(kThreads), 0, stream, f,
@@ -589,13 +661,16 @@ int batch_plan_launch(const BatchPlan* plan, hipStream_t stream) {
 
 void batch_plan_destroy(BatchPlan* plan) {
   if (!plan) return;
+  if (plan->escaped) (void)hipDeviceSynchronize();  // work the allocating stream does not order
   dev_free(plan->mem);
   for (void* p : plan->owned) dev_free(p);
+  tables_release(plan->tables);
   delete plan;
 }
 
 int ntt_plan_create(int count, const uint64_t* const* in, uint64_t* const* out, const int32_t* log_n,
                     const int32_t* inverse, hipStream_t stream, BatchPlan** result) {
+  TableHold hold;
   std::vector<TransformSpec> specs;
   const uint64_t one[4] = {1, 0, 0, 0};
   for (int j = 0; j < count; ++j) {
@@ -612,7 +687,9 @@ int ntt_plan_create(int count, const uint64_t* const* in, uint64_t* const* out, 
     }
     specs.push_back(t);
   }
-  return batch_plan_create(specs, stream, result);
+  if (int rc = batch_plan_create(specs, stream, result)) return rc;
+  (*result)->tables = hold.take();
+  return 0;
 }
 
 }  // namespace zk
@@ -622,6 +699,8 @@ using namespace zk;
 extern "C" {
 
 int zk_ntt_cache_clear(void) { return ntt_cache_clear(); }
+
+int zk_ntt_cache_stats(uint64_t* tables, uint64_t* bytes, uint64_t* pinned) { return ntt_cache_stats(tables, bytes, pinned); }
 
 int zk_ntt_set_fusion(int log_n, int max_stages) {
   if (log_n < 1 || log_n > 28) return fail("log_n must be in [1, 28]");
@@ -670,6 +749,7 @@ int zk_ntt(int field_id, const uint64_t* in, uint64_t* out, int log_n, int inver
     if (in != out) ZK_HIP(hipMemcpyAsync(out, in, 32, hipMemcpyDeviceToDevice, stream));
     return 0;
   }
+  TableHold hold;  // the tables stay pinned until the passes are enqueued
   const uint64_t* tw = nullptr;
   if (int rc = twiddles(field_id, log_n, omega, stream, &tw)) return rc;
   const uint64_t* pre = nullptr;

@@ -28,6 +28,27 @@ int scale_row(int field_id, int log_n, const uint64_t extra[4], hipStream_t stre
 // One row holding the plain constant value, in the 261-form.
 int const_row(int field_id, const uint64_t value[4], hipStream_t stream, const uint64_t** out);
 int ntt_cache_clear();
+int ntt_cache_stats(uint64_t* tables, uint64_t* bytes, uint64_t* pinned);
+
+// Pins on cached tables. While a TableHold lives on a thread, every table that thread obtains is
+// pinned (a use count) until the hold ends or hands its pins to a plan (take()); pinned tables
+// survive ntt_cache_clear and the cache's eviction, so plans and graphs recorded from them never
+// read freed memory, and one-shot calls keep their tables until their kernels are enqueued.
+class TableHold {
+ public:
+  TableHold();
+  ~TableHold();
+  std::vector<const uint64_t*> take();
+  void pin(const uint64_t* table) { pins_.push_back(table); }
+  TableHold(const TableHold&) = delete;
+  TableHold& operator=(const TableHold&) = delete;
+
+ private:
+  std::vector<const uint64_t*> pins_;
+  TableHold* prev_;
+};
+// Releases pins handed over by TableHold::take.
+void tables_release(const std::vector<const uint64_t*>& pins);
 
 // Runs every pass of one transform: pre (n rows) multiplies the input, post
 // (n rows if post_table, else one row) multiplies the output.
@@ -61,13 +82,16 @@ struct BatchPlan {
   std::vector<char> host_desc;       // host copy of every descriptor (source of the upload)
   void* mem = nullptr;
   std::vector<void*> owned;
+  std::vector<const uint64_t*> tables;  // cached tables the descriptors point to (pinned)
+  hipStream_t alloc_stream = nullptr;   // the stream whose allocator cache the memory returns to
+  bool escaped = false;                 // launched on another stream or recorded into a graph
   size_t bytes = 0;                  // device bytes held, owned blocks included
 };
 
 // Scratch rows one transform of size 2^log_n needs between its passes (0 for a single pass).
 int64_t transform_scratch_rows(int log_n);
 int batch_plan_create(const std::vector<TransformSpec>& specs, hipStream_t stream, BatchPlan** out);
-int batch_plan_launch(const BatchPlan* plan, hipStream_t stream);
+int batch_plan_launch(BatchPlan* plan, hipStream_t stream);
 void batch_plan_destroy(BatchPlan* plan);
 
 // The standard primitive 2^log_n-th root of unity of Fr (generator 5) and its inverse, plain.

@@ -60,6 +60,7 @@ int quotient_tables(int log_n, const uint64_t shift[4], hipStream_t stream, Quot
 
 int groth16_quotient(const uint64_t* a, const uint64_t* b, const uint64_t* c, uint64_t* h, int log_n,
                      const uint64_t shift[4], hipStream_t stream) {
+  TableHold hold;  // the tables stay pinned until every pass is enqueued
   QuotientTables t;
   if (int rc = quotient_tables(log_n, shift, stream, &t)) return rc;
   const int64_t n = 1LL << log_n;
@@ -97,6 +98,7 @@ uint64_t quotient_batch_bytes(int count, const int32_t* log_n) {
 int quotient_plan_create(int count, const uint64_t* const* a, const uint64_t* const* b, const uint64_t* const* c,
                          uint64_t* const* h, const int32_t* log_n, const uint64_t* shifts, hipStream_t stream,
                          BatchPlan** out) {
+  TableHold hold;  // handed to the plan, which keeps its tables pinned until destroyed
   std::vector<QuotientTables> tabs((size_t)std::max(count, 0));
   size_t ev_rows = 0;
   for (int j = 0; j < count; ++j) {
@@ -155,6 +157,7 @@ int quotient_plan_create(int count, const uint64_t* const* a, const uint64_t* co
     plan->owned.push_back(ev);
     plan->bytes += ev_rows * 32;
   }
+  plan->tables = hold.take();
   *out = plan;
   return 0;
 }

@@ -146,6 +146,12 @@ void prof_commit(const char* name, hipEvent_t start, hipEvent_t stop) {
   g_prof.push_back(ProfRecord{name, start, stop});
 }
 
+bool launch_escapes(hipStream_t alloc_stream, hipStream_t stream) {
+  if (stream != alloc_stream) return true;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone;
+}
+
 static int g_cus = 0;
 int device_cus() { return g_cus > 0 ? g_cus : 256; }
 
@@ -290,21 +296,23 @@ int zk_stream_create(void** stream) {
 
 int zk_stream_destroy(void* stream) {
   if (!stream) return 0;
+  // Its work finishes first: the cached blocks may still be read by it.
+  ZK_HIP(hipStreamSynchronize((hipStream_t)stream));
   {
-    // Blocks cached for this stream can no longer be ordered by it.
+    // Blocks of this stream, cached or live, are ordered by the null stream from now on (a later
+    // stream may get the same handle).
     std::lock_guard<std::mutex> lk(g_alloc_mu);
     for (auto it = g_free.begin(); it != g_free.end();) {
       if (it->first.first == (hipStream_t)stream) {
-        for (void* p : it->second) {
-          g_free[{nullptr, it->first.second}].push_back(p);
-        }
+        for (void* p : it->second) g_free[{nullptr, it->first.second}].push_back(p);
         it = g_free.erase(it);
       } else {
         ++it;
       }
     }
+    for (auto& kv : g_live)
+      if (kv.second.stream == (hipStream_t)stream) kv.second.stream = nullptr;
   }
-  ZK_HIP(hipStreamSynchronize((hipStream_t)stream));
   ZK_HIP(hipStreamDestroy((hipStream_t)stream));
   return 0;
 }

@@ -42,6 +42,11 @@ class Scratch {
 
 int device_cus();  // compute units of the current device
 
+// Whether work on `stream` may outlive what the plan's allocating stream orders: the launch runs on
+// another stream or is being recorded into a graph (which may be replayed on any stream). A plan
+// that answered yes waits for the device before its memory returns to the allocator's cache.
+bool launch_escapes(hipStream_t alloc_stream, hipStream_t stream);
+
 // Opt-in kernel timing (zk_prof_enable): launch_timed passes a start and a stop event to
 // hipExtLaunchKernelGGL, so the dispatch itself takes the timestamps (marker packets between
 // kernels would add a gap of about 10 us each); zk_prof_read sums the durations per name.

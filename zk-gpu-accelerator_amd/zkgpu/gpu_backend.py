@@ -102,6 +102,7 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_field_sub": [_i32, _vp, _i32, _vp, _i32, _vp, _i64, _vp],
     "zk_ntt": [_i32, _vp, _vp, _i32, _i32, _u64p, _u64p, _vp],
     "zk_ntt_cache_clear": [],
+    "zk_ntt_cache_stats": [_u64p, _u64p, _u64p],
     "zk_ntt_set_fusion": [_i32, _i32],
     "zk_ntt_get_fusion": [_i32, ctypes.POINTER(_i32)],
     "zk_groth16_quotient": [_vp, _vp, _vp, _vp, _i32, _u64p, _vp],
