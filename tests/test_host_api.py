@@ -138,3 +138,32 @@ def test_no_cpu_fallback():
     ):
         with pytest.raises(RuntimeError, match="no CPU fallback"):
             call()
+
+
+def test_bulk_int_rows_match_python_conversion():
+    """The C ingest helper (zkgpu._ingest) gives the rows int.to_bytes gives, reduces only values
+    outside [0, 2^256) on the host, accepts what operator.index accepts, and raises the
+    reference's TypeError (msm_gpu.py:385-390)."""
+    import random
+
+    import numpy as np
+
+    from zkgpu import field256_gpu as fg
+
+    assert fg._ingest is not None, "zkgpu._ingest is not built"
+    r = fg.BN254_SCALAR_MODULUS
+    rng = random.Random(3)
+    vals = [rng.getrandbits(256) for _ in range(500)] + [0, 1, 2**256 - 1, 2**256, -1, -(2**300), r, True]
+    vals.append(np.int64(-7))
+    rows, items = fg._rows_from_ints(iter(vals), r, "scalars")
+    want = [v if 0 <= v < 2**256 else int(v) % r for v in (int(x) for x in vals)]
+    assert fg._limbs_to_ints(rows) == want
+    assert list(items) == vals
+    assert rows.shape == (len(vals), 4) and rows.dtype == np.dtype("<u8")
+    back = b"".join(v.to_bytes(32, "little") for v in want)
+    assert rows.tobytes() == back
+    with pytest.raises(TypeError, match=r"^scalars must contain integers: 'float' object cannot be interpreted"):
+        fg._rows_from_ints([1, 2.5], r, "scalars")
+    with pytest.raises(TypeError, match=r"^'str' object cannot be interpreted as an integer"):
+        fg._rows_from_ints([1, "2"], r)
+    assert fg._rows_from_ints([], r)[0].shape == (0, 4)

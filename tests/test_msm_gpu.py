@@ -9,14 +9,16 @@ P_i = k_i G, checked against the closed form (sum s_i k_i mod r) G.
 """
 
 import random
+import sys
 
 import numpy as np
 import pytest
 
-from zkgpu import gpu_backend, msm_gpu, workloads
+from zkgpu import gpu_backend, gpu_backend as gb, msm_gpu, workloads
 from zkgpu.curve import Point as gPoint
 from zkgpu.field256_gpu import BN254_FQ
 from zkgpu.msm_gpu import msm_g1, msm_pippenger_gpu
+from zkref import fast
 from zkref.curve import G, P, Point
 from zkref.field import R_SCALAR as R, ints_to_limbs, limbs_to_ints
 from zkref.msm import msm_double_and_add, msm_naive, msm_pippenger
@@ -269,3 +271,59 @@ def test_run_sums_that_cancel_across_ranges(chunk):
         msm_gpu.msm_g1_jacobian(pts, gpu_backend.asarray(ints_to_limbs(scalars)), out, chunk_len=chunk)
         got = msm_gpu._result(out)
         assert (got.is_identity() and e == 0) or got.affine() == (G * e).affine(), (chunk, scalars[-1])
+
+
+@pytest.fixture(scope="module")
+def basis20():
+    n = 1 << 20
+    keys = workloads.key_rows(n, 0x5EED04)
+    points = workloads.srs_points(keys)
+    scalars = workloads.witness_rows_fast(n, 0x5EED31)
+    e = fast.dot_mod_r(keys, scalars)
+    return points, scalars, (None if e == 0 else (G * e).affine())
+
+
+def test_public_call_at_2p20_with_python_ints(basis20):
+    """msm_pippenger_gpu with Python ints at 2^20 (bulk conversion, device reduction and on-curve
+    check) equals the closed form (sum k_i s_i mod r) G."""
+    import time
+
+    points, scalars, want = basis20
+    coords = BN254_FQ.to_ints(points.reshape(-1, 4))
+    xs, ys = coords[0::2], coords[1::2]
+    sc = limbs_to_ints(scalars)
+    t0 = time.perf_counter()
+    got = msm_pippenger_gpu(xs, ys, sc)
+    print(f"public call, 2^20 Python ints: {(time.perf_counter() - t0) * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    assert got.affine() == want
+
+
+def test_public_call_with_resident_points(basis20):
+    """The same signature over device-resident inputs: an SRS (with and without window tables) or
+    (n, 2, 4) rows in HBM as points_x, scalars as device or NumPy rows."""
+    from zkgpu.batch import SRS
+
+    points, scalars, want = basis20
+    dev_sc = gb.asarray(scalars)
+    assert msm_pippenger_gpu(points, None, dev_sc).affine() == want
+    assert msm_pippenger_gpu(points, None, scalars).affine() == want
+    srs = SRS(points)
+    assert msm_pippenger_gpu(srs, None, dev_sc).affine() == want
+    srs.precompute((16,))
+    assert msm_pippenger_gpu(srs, None, dev_sc).affine() == want
+    # an MSM over a prefix of the SRS
+    keys = workloads.key_rows(1000, 0x5EED04)
+    e = fast.dot_mod_r(keys, scalars[:1000])
+    assert msm_pippenger_gpu(srs, None, dev_sc[:1000]).affine() == (G * e).affine()
+    with pytest.raises(ValueError, match="Length mismatch"):
+        msm_pippenger_gpu(points[:10], None, dev_sc[:11])
+    with pytest.raises(TypeError, match="points_y must be None"):
+        msm_pippenger_gpu(points, [1], dev_sc)
+    assert msm_pippenger_gpu(srs, None, dev_sc[:0]).is_identity()
+
+
+def test_resident_points_off_the_curve_are_refused():
+    bad = np.zeros((3, 2, 4), dtype=np.uint64)
+    pts = BN254_FQ.from_limbs(bad.reshape(-1, 4)).reshape(3, 2, 4)  # (0, 0) is not on the curve
+    with pytest.raises(ValueError, match=r"\(0, 0\) is not on the BN254 curve"):
+        msm_pippenger_gpu(pts, None, gb.asarray(np.ones((3, 4), dtype=np.uint64)))

@@ -25,7 +25,7 @@ from __future__ import annotations
 
 import ctypes
 import operator
-from collections.abc import Iterable
+from collections.abc import Iterable, Sequence
 from pathlib import Path
 from typing import Any
 
@@ -43,13 +43,43 @@ _R_BITS = LIMBS * _LIMB_BITS  # R = 2^256
 _CSRC = Path(__file__).resolve().parent.parent / "csrc"
 
 
+try:  # bulk int <-> row conversion in C (csrc/zk_ingest.c); host bookkeeping only
+    from zkgpu import _ingest
+except ImportError:  # not built: the same conversions one element at a time
+    _ingest = None
+
+
 def _ints_to_limbs(values: Iterable[int]) -> np.ndarray:
     """(n, 4) uint64 little-endian limbs of values in [0, 2^256)."""
-    buf = b"".join(v.to_bytes(32, "little") for v in values)
+    if _ingest is not None:
+        buf, _ = _ingest.rows_from_ints(values, 2**_R_BITS, "")
+    else:
+        buf = b"".join(v.to_bytes(32, "little") for v in values)
     return np.frombuffer(buf, dtype="<u8").astype(np.uint64).reshape(-1, LIMBS)
 
 
+def _rows_from_ints(values: Iterable[Any], modulus: int, name: str = "") -> tuple[np.ndarray, Sequence[Any]]:
+    """(n, 4) uint64 rows of integers (anything operator.index accepts) and the sequence they were
+    read from. Values in [0, 2^256) are kept as they are (the device reduces them); others are
+    reduced mod `modulus` here. A non-integer raises TypeError: operator.index's, or with a name,
+    "<name> must contain integers: ..." (msm_gpu.py:385-390)."""
+    if _ingest is not None:
+        buf, items = _ingest.rows_from_ints(values, modulus, name)
+        return np.frombuffer(buf, dtype="<u8").reshape(-1, LIMBS), items
+    items = values if isinstance(values, Sequence) else list(values)
+    try:
+        ints = [operator.index(v) for v in items]
+    except TypeError as exc:
+        if not name:
+            raise
+        raise TypeError(f"{name} must contain integers: {exc}") from None
+    kept = [v if 0 <= v < 2**_R_BITS else v % modulus for v in ints]
+    return _ints_to_limbs(kept), items
+
+
 def _limbs_to_ints(limbs: np.ndarray) -> list[int]:
+    if _ingest is not None:
+        return _ingest.ints_from_rows(np.ascontiguousarray(limbs, dtype="<u8"))
     buf = np.ascontiguousarray(limbs, dtype="<u8").tobytes()
     return [int.from_bytes(buf[k : k + 32], "little") for k in range(0, len(buf), 32)]
 
@@ -104,12 +134,12 @@ class Field256:
     # ------------------------------------------------------------------
 
     def from_ints(self, values: Iterable[int]) -> DeviceArray:
-        """Montgomery-form rows for any ints (reduced mod the modulus first)."""
-        reduced = [operator.index(v) % self.modulus for v in values]
+        """Montgomery-form rows for any ints (reduced mod the modulus: negative values and those of
+        2^256 and more on the host, the rest by the conversion kernel)."""
+        rows, _ = _rows_from_ints(values, self.modulus)
         _gb.require_gpu("Field256.from_ints", _gpu_available)
         fid = self._compile()["id"]
-        plain = _gb.asarray(_ints_to_limbs(reduced))
-        return self._to_mont(fid, plain)
+        return self._to_mont(fid, _gb.asarray(np.ascontiguousarray(rows, dtype=np.uint64)))
 
     def from_limbs(self, plain: Any) -> DeviceArray:
         """Montgomery-form rows for plain (n, 4) uint64 limbs (host or device), each < 2^256."""

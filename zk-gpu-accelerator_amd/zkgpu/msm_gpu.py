@@ -22,7 +22,7 @@ from __future__ import annotations
 
 import ctypes
 import operator
-from collections.abc import Iterable
+from collections.abc import Iterable, Sequence
 from typing import Any, SupportsIndex
 
 import numpy as np
@@ -30,7 +30,7 @@ import numpy as np
 from zkgpu import gpu_backend as _gb
 from zkgpu.curve import P as _P, Point, from_jacobian
 from zkgpu.field import FIELD_MODULUS as _GROUP_ORDER
-from zkgpu.field256_gpu import BN254_FQ, LIMBS as _FIELD_LIMBS, _ints_to_limbs, _limbs_to_ints
+from zkgpu.field256_gpu import BN254_FQ, LIMBS as _FIELD_LIMBS, _limbs_to_ints, _rows_from_ints
 from zkgpu.gpu_backend import GPU_AVAILABLE as _gpu_available, DeviceArray
 from zkgpu.msm import _check_window_size
 
@@ -97,27 +97,85 @@ def _run(points: DeviceArray, scalars: DeviceArray, n: int, window: int, chunk: 
     return out
 
 
-def _check_points(points: DeviceArray, xs: list[int], ys: list[int]) -> None:
+def _check_points(points: DeviceArray, xs: Sequence[Any], ys: Sequence[Any]) -> None:
+    """ValueError naming the first off-curve point (msm_gpu.py:441-442 checks each Point on the host)."""
     bad = ctypes.c_int64(-1)
     _gb.call("zk_g1_on_curve", points.ptr, points.shape[0], ctypes.byref(bad), _gb.current_stream())
     if bad.value >= 0:
         i = bad.value
-        raise ValueError(f"({xs[i] % _P}, {ys[i] % _P}) is not on the BN254 curve y^2 = x^3 + 3")
+        x, y = (operator.index(xs[i]) % _P, operator.index(ys[i]) % _P) if xs is not None else \
+            tuple(BN254_FQ.to_ints(points[i]))
+        raise ValueError(f"({x}, {y}) is not on the BN254 curve y^2 = x^3 + 3")
 
 
-def _msm_ints(xs: list[int], ys: list[int], reduced: list[int], window: int, chunk: int) -> Point:
-    n = len(xs)
-    coords = _ints_to_limbs(c % _P for xy in zip(xs, ys) for c in xy)
-    points = BN254_FQ.from_limbs(coords).reshape(n, 2, _FIELD_LIMBS)
+def _msm_rows(xrows: np.ndarray, yrows: np.ndarray, srows: np.ndarray, xs: Sequence[Any], ys: Sequence[Any],
+              window: int, chunk: int) -> Point:
+    """The MSM of host rows: coordinates below 2^256 (reduced mod p on the device), scalars below
+    2^256 (reduced mod r on the device)."""
+    n = xrows.shape[0]
+    coords = np.empty((n, 2, _FIELD_LIMBS), dtype=np.uint64)
+    coords[:, 0] = xrows
+    coords[:, 1] = yrows
+    points = BN254_FQ.from_limbs(coords.reshape(-1, _FIELD_LIMBS)).reshape(n, 2, _FIELD_LIMBS)
     _check_points(points, xs, ys)
-    scalars = _gb.asarray(_ints_to_limbs(reduced))
+    scalars = _gb.asarray(np.ascontiguousarray(srows, dtype=np.uint64))
     return _result(_run(points, scalars, n, window, chunk))
 
 
+def _msm_ints(xs: list[int], ys: list[int], reduced: list[int], window: int, chunk: int) -> Point:
+    xrows, xs = _rows_from_ints(xs, _P)
+    yrows, ys = _rows_from_ints(ys, _P)
+    srows, _ = _rows_from_ints(reduced, _GROUP_ORDER)
+    return _msm_rows(xrows, yrows, srows, xs, ys, window, chunk)
+
+
+def _scalar_rows(scalars: Any) -> DeviceArray:
+    """Scalars as (n, 4) uint64 device rows: a DeviceArray or NumPy array of plain limbs as it is,
+    anything else read as integers (reduced mod r)."""
+    if isinstance(scalars, DeviceArray) or (isinstance(scalars, np.ndarray) and scalars.dtype == np.uint64):
+        if scalars.ndim != 2 or scalars.shape[1] != _FIELD_LIMBS:
+            raise TypeError(f"scalars must be an (n, 4) uint64 array of plain limbs, got shape {scalars.shape}")
+        return _gb.asarray(np.ascontiguousarray(scalars) if isinstance(scalars, np.ndarray) else scalars)
+    rows, _ = _rows_from_ints(scalars, _GROUP_ORDER, "scalars")
+    return _gb.asarray(np.ascontiguousarray(rows, dtype=np.uint64))
+
+
+def _msm_resident(points: Any, points_y: Any, scalars: Any, window: int) -> Point:
+    """msm_pippenger_gpu over device-resident points: an SRS (an MSM over its first n points, with
+    its window tables) or (n, 2, 4) Montgomery rows already in HBM."""
+    from zkgpu.batch import SRS
+
+    if points_y is not None:
+        raise TypeError("points_y must be None when points_x holds device-resident points (an SRS or (n, 2, 4) rows)")
+    _gb.require_gpu("msm_pippenger_gpu", _gpu_available)
+    _kernels()
+    sc = _scalar_rows(scalars)
+    n = int(sc.shape[0])
+    if isinstance(points, SRS):
+        if n > points.n:
+            raise ValueError(f"Length mismatch: an SRS of {points.n} points, {n} scalars.")
+        if n == 0:
+            return Point.identity()
+        out = _gb.empty((3, _FIELD_LIMBS))
+        ptrs = (ctypes.c_void_p * 1)(sc.ptr)
+        sizes = (ctypes.c_int64 * 1)(n)
+        wins = (ctypes.c_int32 * 1)(window)
+        _gb.call("zk_msm_g1_batch", points.handle, 1, ptrs, sizes, wins, out.ptr, _gb.current_stream())
+        return _result(out)
+    if points.ndim != 3 or tuple(points.shape[1:]) != (2, _FIELD_LIMBS) or points.dtype != np.uint64:
+        raise TypeError(f"points must be an (n, 2, 4) uint64 array, got {points.shape}")
+    if points.shape[0] != n:
+        raise ValueError(f"Length mismatch: {points.shape[0]} points, {n} scalars.")
+    if n == 0:
+        return Point.identity()
+    _check_points(points, None, None)
+    return _result(_run(points, sc, n, window, _CHUNK_LEN))
+
+
 def msm_pippenger_gpu(
-    points_x: Iterable[int],
-    points_y: Iterable[int],
-    scalars: Iterable[int],
+    points_x: Iterable[int] | Any,
+    points_y: Iterable[int] | None,
+    scalars: Iterable[int] | Any,
     *,
     window_size: int | None = None,
 ) -> Point:
@@ -126,8 +184,11 @@ def msm_pippenger_gpu(
 
     Parameters
     ----------
-    points_x, points_y : affine coordinates (Python ints or an object array)
-    scalars            : any integers; reduced mod r, which leaves every s_i * P_i unchanged
+    points_x, points_y : affine coordinates (Python ints or an object array); or points_x a
+                         device-resident basis (a zkgpu.batch.SRS, summed over its first n points,
+                         or (n, 2, 4) Montgomery rows in HBM) with points_y None
+    scalars            : any integers, reduced mod r, which leaves every s_i * P_i unchanged;
+                         or (n, 4) uint64 plain limbs (a DeviceArray or NumPy array)
     window_size        : bits per signed digit, 1..16 (default: this GPU's autotuned
                          window, else the device cost model)
 
@@ -140,17 +201,32 @@ def msm_pippenger_gpu(
     """
     if window_size is not None:
         window_size = _check_window_size(window_size)
-    xs, ys = _as_ints(points_x, "points_x"), _as_ints(points_y, "points_y")
-    reduced = [s % _GROUP_ORDER for s in _as_ints(scalars, "scalars")]
-    if not len(xs) == len(ys) == len(reduced):
+    from zkgpu.batch import SRS
+
+    if isinstance(points_x, (SRS, DeviceArray)):
+        return _msm_resident(points_x, points_y, scalars, window_size or 0)
+    xrows, xs = _rows_from_ints(points_x, _P, "points_x")
+    yrows, ys = _rows_from_ints(points_y, _P, "points_y")
+    if isinstance(scalars, (DeviceArray, np.ndarray)) and getattr(scalars, "dtype", None) == np.uint64:
+        srows = scalars
+    else:
+        srows, _ = _rows_from_ints(scalars, _GROUP_ORDER, "scalars")
+    if not xrows.shape[0] == yrows.shape[0] == srows.shape[0]:
         raise ValueError(
-            f"Length mismatch: {len(xs)} x-coordinates, {len(ys)} y-coordinates, {len(reduced)} scalars."
+            f"Length mismatch: {xrows.shape[0]} x-coordinates, {yrows.shape[0]} y-coordinates, {srows.shape[0]} scalars."
         )
-    if not reduced:
+    if xrows.shape[0] == 0:
         return Point.identity()
     _gb.require_gpu("msm_pippenger_gpu", _gpu_available)
     _kernels()
-    return _msm_ints(xs, ys, reduced, window_size or 0, _CHUNK_LEN)
+    if isinstance(srows, DeviceArray):
+        n = xrows.shape[0]
+        coords = np.empty((n, 2, _FIELD_LIMBS), dtype=np.uint64)
+        coords[:, 0], coords[:, 1] = xrows, yrows
+        points = BN254_FQ.from_limbs(coords.reshape(-1, _FIELD_LIMBS)).reshape(n, 2, _FIELD_LIMBS)
+        _check_points(points, xs, ys)
+        return _result(_run(points, srows, n, window_size or 0, _CHUNK_LEN))
+    return _msm_rows(xrows, yrows, srows, xs, ys, window_size or 0, _CHUNK_LEN)
 
 
 def msm_g1(points: Any, scalars: Any, *, window_size: int | None = None) -> Point:
