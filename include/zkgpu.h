@@ -232,6 +232,43 @@ int zk_msm_plan_bytes(void* plan, uint64_t* bytes);
 int zk_msm_g1_batch(void* srs, int count, const uint64_t* const* scalars, const int64_t* n,
                     const int32_t* window, uint64_t* out_jac, void* stream);
 
+/* ---- 64-bit prime fields: int64 elements modulo 2 <= mod < 2^63 ----
+ * Operands are device arrays of any integer dtype (codes below), widened to int64 as NumPy's
+ * astype(int64) does (uint64 keeps its bit pattern) and floor-reduced into [0, mod) on load.
+ * Broadcasting is by strides: shape has ndim <= 8 dimensions (the output's, C order), strides
+ * are in elements per dimension and 0 where an operand is broadcast. Outputs are contiguous
+ * int64 in [0, mod). */
+#define ZK_I8 0
+#define ZK_I16 1
+#define ZK_I32 2
+#define ZK_I64 3
+#define ZK_U8 4
+#define ZK_U16 5
+#define ZK_U32 6
+#define ZK_U64 7
+/* out = a * b (op 0) or a + b (op 1) mod mod. Replaces the zk_modmul / zk_modadd elementwise
+ * kernels (src/zkgpu/field_gpu.py:73-101) behind gpu_batch_mul (:147) and gpu_batch_add (:181). */
+int zk_f64_binary(int op, const void* a, int a_dtype, const int64_t* a_strides, const void* b, int b_dtype,
+                  const int64_t* b_strides, int ndim, const int64_t* shape, int64_t* out, uint64_t mod,
+                  void* stream);
+/* out = base^e mod mod for e >= 0 given as exp_len little-endian 64-bit words (e = 0 gives 1).
+ * Replaces the launch-per-bit square-and-multiply of _gpu_pow_nonnegative
+ * (src/zkgpu/field_gpu.py:135-144) behind gpu_batch_pow (:199). */
+int zk_f64_pow(const void* base, int dtype, const int64_t* strides, int ndim, const int64_t* shape,
+               const uint64_t* exp_words, int exp_len, int64_t* out, uint64_t mod, void* stream);
+/* *result = 1 if some element of base is 0 mod mod, else 0; synchronises stream. Replaces the
+ * invertibility check of gpu_batch_pow for negative exponents (src/zkgpu/field_gpu.py:221-223). */
+int zk_f64_any_zero(const void* base, int dtype, const int64_t* strides, int ndim, const int64_t* shape,
+                    uint64_t mod, int* result, void* stream);
+/* The NTT X_k = sum_j in_j omega^(jk) of 2^log_n values (log_n <= 33) modulo an odd prime
+ * mod < 2^63, omega a primitive 2^log_n-th root of unity; invert uses omega^-1 and scales by
+ * n^-1. in (contiguous, any integer dtype, reduced on load) is left unchanged; out (contiguous
+ * int64, natural order) must be another buffer. Twiddle tables are cached per (mod, omega,
+ * log_n, invert) with the other NTT tables. Replaces ntt_gpu's per-stage CuPy butterflies,
+ * bit-reversal gather and cached twiddle table (src/zkgpu/ntt_gpu.py:48-155). */
+int zk_ntt64(const void* in, int dtype, int64_t* out, int log_n, uint64_t mod, uint64_t omega, int invert,
+             void* stream);
+
 #ifdef __cplusplus
 }
 #endif

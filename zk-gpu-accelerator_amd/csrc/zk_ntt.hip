@@ -372,20 +372,33 @@ int expand(int field_id, const HostField& hf, const U256& base_plain, int log_co
   return 0;
 }
 
+// A cached table's address (moved to the front of the LRU list and pinned by the thread's hold), or
+// false if the key is not cached. Called with g_tab_mu held.
+bool lookup_locked(const TableKey& key, const uint64_t** out) {
+  auto it = g_tables.find(key);
+  if (it == g_tables.end()) return false;
+  g_lru.remove(key);
+  g_lru.push_front(key);
+  if (g_hold) {
+    ++it->second.uses;
+    g_hold->pin(it->second.ptr);
+  }
+  *out = it->second.ptr;
+  return true;
+}
+
+// Adds a freshly built table (g_tab_mu held).
+void insert_locked(const TableKey& key, uint64_t* buf, size_t rows, size_t bytes) {
+  g_tables[key] = Table{buf, rows, bytes, g_hold ? 1 : 0};
+  g_table_bytes += bytes;
+  g_lru.push_front(key);
+  if (g_hold) g_hold->pin(buf);
+}
+
 // Returns a cached table, building it on first use.
 int get_table(const TableKey& key, hipStream_t stream, const uint64_t** out) {
   std::lock_guard<std::mutex> lk(g_tab_mu);
-  auto it = g_tables.find(key);
-  if (it != g_tables.end()) {
-    g_lru.remove(key);
-    g_lru.push_front(key);
-    if (g_hold) {
-      ++it->second.uses;
-      g_hold->pin(it->second.ptr);
-    }
-    *out = it->second.ptr;
-    return 0;
-  }
+  if (lookup_locked(key, out)) return 0;
   U256 p;
   if (!field_modulus(key.field, &p)) return fail("unknown field id");
   HostField hf(p);
@@ -422,10 +435,7 @@ int get_table(const TableKey& key, hipStream_t stream, const uint64_t** out) {
     (void)hipFree(buf);
     return rc;
   }
-  g_tables[key] = Table{buf, rows, bytes, g_hold ? 1 : 0};
-  g_table_bytes += bytes;
-  g_lru.push_front(key);
-  if (g_hold) g_hold->pin(buf);
+  insert_locked(key, buf, rows, bytes);
   *out = buf;
   return 0;
 }
@@ -523,6 +533,28 @@ int scale_row(int field_id, int log_n, const uint64_t extra[4], hipStream_t stre
 int const_row(int field_id, const uint64_t value[4], hipStream_t stream, const uint64_t** out) {
   TableKey key{field_id, 0, 4, {value[0], value[1], value[2], value[3]}, {0, 0, 0, 0}};
   return get_table(key, stream, out);
+}
+
+int host_table(int log_n, const uint64_t tag[4], size_t words, const std::function<void(uint64_t*)>& build,
+               hipStream_t stream, const uint64_t** out) {
+  TableKey key{-1, log_n, 5, {tag[0], tag[1], tag[2], tag[3]}, {0, 0, 0, 0}};
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  if (lookup_locked(key, out)) return 0;
+  std::vector<uint64_t> host(words);
+  build(host.data());
+  const size_t bytes = words * 8;
+  if (int rc = evict_locked(kTableCacheBytes > bytes ? kTableCacheBytes - bytes : 0)) return rc;
+  uint64_t* buf = nullptr;
+  ZK_HIP(hipMalloc(&buf, bytes));
+  hipError_t err = hipMemcpyAsync(buf, host.data(), bytes, hipMemcpyHostToDevice, stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(stream);  // the host copy is released on return
+  if (err != hipSuccess) {
+    (void)hipFree(buf);
+    return fail_hip(err, "table upload");
+  }
+  insert_locked(key, buf, words, bytes);
+  *out = buf;
+  return 0;
 }
 
 // Descriptors of every pass of one transform (tmp: n rows of scratch when there are several passes).

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <vector>
 
 namespace zk {
@@ -27,6 +28,10 @@ int coset_table(int field_id, int log_n, const uint64_t g[4], bool inverse, cons
 int scale_row(int field_id, int log_n, const uint64_t extra[4], hipStream_t stream, const uint64_t** out);
 // One row holding the plain constant value, in the 261-form.
 int const_row(int field_id, const uint64_t value[4], hipStream_t stream, const uint64_t** out);
+// A table of `words` 64-bit words that `build` fills on the host on first use, then uploaded and
+// cached like the others; tag (with log_n) names it.
+int host_table(int log_n, const uint64_t tag[4], size_t words, const std::function<void(uint64_t*)>& build,
+               hipStream_t stream, const uint64_t** out);
 int ntt_cache_clear();
 int ntt_cache_stats(uint64_t* tables, uint64_t* bytes, uint64_t* pinned);
 

@@ -103,6 +103,10 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_ntt": [_i32, _vp, _vp, _i32, _i32, _u64p, _u64p, _vp],
     "zk_ntt_cache_clear": [],
     "zk_ntt_cache_stats": [_u64p, _u64p, _u64p],
+    "zk_f64_binary": [_i32, _vp, _i32, _i64p, _vp, _i32, _i64p, _i32, _i64p, _vp, _u64, _vp],
+    "zk_f64_pow": [_vp, _i32, _i64p, _i32, _i64p, _u64p, _i32, _vp, _u64, _vp],
+    "zk_f64_any_zero": [_vp, _i32, _i64p, _i32, _i64p, _u64, ctypes.POINTER(_i32), _vp],
+    "zk_ntt64": [_vp, _i32, _vp, _i32, _u64, _u64, _i32, _vp],
     "zk_ntt_set_fusion": [_i32, _i32],
     "zk_ntt_get_fusion": [_i32, ctypes.POINTER(_i32)],
     "zk_groth16_quotient": [_vp, _vp, _vp, _vp, _i32, _u64p, _vp],
@@ -469,7 +473,9 @@ def asarray(host: Any, dtype: Any = None) -> DeviceArray:
     """Upload a host array (a DeviceArray is returned unchanged)."""
     if isinstance(host, DeviceArray):
         return host
-    arr = np.ascontiguousarray(host, dtype=dtype)
+    arr = np.asarray(host, dtype=dtype)
+    if not arr.flags.c_contiguous:
+        arr = arr.copy(order="C")  # (np.ascontiguousarray would also turn a 0-d array into 1-d)
     out = DeviceArray(arr.shape, arr.dtype)
     if out.nbytes:
         call("zk_memcpy_h2d", out.ptr, arr.ctypes.data, out.nbytes, current_stream())
