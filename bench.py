@@ -42,26 +42,6 @@ HBM_PEAK_GBPS = 8000.0
 SRS_TABLES = (12, 16)  # window tables of the batched/served SRS
 
 
-def signed_digit_count(scalars: np.ndarray, c: int) -> int:
-    """Nonzero signed c-bit digits over all scalars (the accumulation's unit count)."""
-    windows = 254 // c + 1
-    s = np.ascontiguousarray(scalars, dtype=np.uint64)
-    carry = np.zeros(s.shape[0], dtype=np.int64)
-    half = 1 << (c - 1)
-    total = 0
-    for w in range(windows):
-        bit = w * c
-        word, sh = bit // 64, bit % 64
-        raw = (s[:, word] >> np.uint64(sh)) if word < 4 else np.zeros(s.shape[0], dtype=np.uint64)
-        if sh + c > 64 and word + 1 < 4:
-            raw = raw | (s[:, word + 1] << np.uint64(64 - sh))
-        raw = (raw & np.uint64((1 << c) - 1)).astype(np.int64) + carry
-        carry = (raw > half).astype(np.int64)
-        digit = raw - carry * (1 << c)
-        total += int(np.count_nonzero(digit))
-    return total
-
-
 def timed(fn, steps: int, warmup: int, gb) -> float:
     """Milliseconds per step over exactly `steps` calls, synchronised on both sides."""
     for _ in range(warmup):
@@ -91,7 +71,8 @@ def main() -> int:
     ap.add_argument("--ntt-only", action="store_true",
                     help="only the forward NTT at --log-ntt (for the pass kernel's trace and byte counters)")
     ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs of the served workload")
-    ap.add_argument("--serve-rate", type=float, default=300.0, help="Poisson arrival rate, jobs/s")
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="an extra Poisson arrival rate (jobs/s) beside 0.5, 0.8 and 0.95 of the measured capacity")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="HBM budget of the server")
     ap.add_argument("--plain-sample", type=int, default=200, help="jobs timed through the per-call path")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -160,14 +141,14 @@ def main() -> int:
     from zkgpu.autotune import lookup
 
     c_used = window or lookup("msm_window", n) or _default_window(gb, n)
-    entries = signed_digit_count(scalars_host, c_used)
+    entries = _msm_entries(gb, points, scalars, n, window)
     mads_per_entry = 11 * 136
     acc_avg = acc_ms / max(acc_launches, 1)
     achieved_tmads = entries * mads_per_entry / (acc_avg * 1e-3) / 1e12
-    peak_tmads = _valu_peak(gb)
+    peaks = _valu_peaks(gb)
     msm_mpts = n / (ms * 1e-3) / 1e6
-    log(f"msm 2^{args.log_msm}: {ms:.3f} ms/step, {msm_mpts:.2f} Mpts/s; accumulate {acc_avg:.3f} ms "
-        f"({achieved_tmads:.2f} Tmad/s of {peak_tmads:.2f})")
+    log(f"msm 2^{args.log_msm}: {ms:.3f} ms/step, {msm_mpts:.2f} Mpts/s; accumulate {acc_avg:.3f} ms, {entries} entries "
+        f"({achieved_tmads:.2f} Tmad/s; peak {peaks['derived']:.2f} derived, {peaks['measured_max']:.2f} measured)")
 
     traffic = _traffic("k_accumulate", "msm_points", n)
     extras: dict = {}
@@ -211,14 +192,14 @@ def main() -> int:
             "kernel": "k_accumulate",
             "bound": "valu",
             "achieved": round(achieved_tmads, 4),
-            "peak": round(peak_tmads, 4),
+            **_peak_fields(achieved_tmads, peaks),
             "unit": "Tmad/s",
-            "frac": round(achieved_tmads / peak_tmads, 4) if peak_tmads else None,
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
             "traffic_source": traffic.get("source") if traffic else None,
             "algorithmic_bytes_per_launch": entries * 72,
             "avg_launch_ms": round(acc_avg, 4),
             "units_per_launch": entries,
+            "units_source": "the plan's sorted-entry count (zk_msm_g1_entries on the timed inputs)",
             "work_per_unit": "one mixed addition: 11 Fq mul x 136 32-bit mad; 72 bytes (key, point ref, affine point)",
         },
         "cpu_baseline": cpu,
@@ -275,6 +256,49 @@ def _valu_peak(gb) -> float:
     out = ctypes.c_double(0)
     gb.call("zk_valu_peak", ctypes.byref(out))
     return float(out.value)
+
+
+_PEAKS: dict = {}
+
+
+def _valu_peaks(gb, runs: int = 5) -> dict:
+    """The VALU multiply-add peak two ways: derived from the device's compute units and peak engine
+    clock (64 lanes per CU, one v_mad_u64_u32 per lane per clock), and the maximum of `runs`
+    zk_valu_peak measurements (a dependent-chain microbenchmark; it swings with clocks, so the
+    maximum, not one sample). Measured once per process."""
+    import ctypes
+
+    if not _PEAKS:
+        name = ctypes.create_string_buffer(256)
+        cus, clk = ctypes.c_int(), ctypes.c_int()
+        gb.call("zk_device_info", name, 256, ctypes.byref(cus), ctypes.byref(clk))
+        derived = cus.value * 64 * clk.value * 1e3 / 1e12
+        samples = [_valu_peak(gb) for _ in range(runs)]
+        _PEAKS.update(derived=derived, measured_max=max(samples), samples=[round(x, 3) for x in samples],
+                      source=f"{cus.value} CUs x 64 lanes x {clk.value // 1000} MHz; max of {runs} zk_valu_peak runs")
+    return _PEAKS
+
+
+def _peak_fields(achieved: float, peaks: dict) -> dict:
+    return {
+        "peak": round(peaks["derived"], 4),
+        "frac": round(achieved / peaks["derived"], 4),
+        "peak_measured_max": round(peaks["measured_max"], 4),
+        "frac_vs_measured_max": round(achieved / peaks["measured_max"], 4),
+        "peak_samples": peaks["samples"],
+        "peak_source": peaks["source"],
+    }
+
+
+def _msm_entries(gb, points, scalars, n: int, window: int) -> int:
+    """The (bucket, point) entries the accumulation processes for these inputs: the plan's own
+    sorted-entry count, read after one zk_msm_g1_entries call."""
+    import ctypes
+
+    out = gb.empty((3, 4))
+    m = ctypes.c_uint64(0)
+    gb.call("zk_msm_g1_entries", points.ptr, scalars.ptr, n, window, 0, out.ptr, ctypes.byref(m), gb.current_stream())
+    return int(m.value)
 
 
 def _field_mul(args, gb, Fr, workloads, log) -> dict:
@@ -345,9 +369,9 @@ def _ntt_and_pipeline(args, gb, Fr, workloads, log, quotient: bool = True) -> di
     pass_gbps = 64.0 * n / (per_pass * 1e-3) / 1e9  # each pass reads and writes every row once
     # one Montgomery product per butterfly, n/2 butterflies per stage, log n stages
     ntt_tmads = args.log_ntt * (n // 2) * 136 / (ntt_pass_ms * 1e-3) / 1e12
-    peak = _valu_peak(gb)
+    peaks = _valu_peaks(gb)
     log(f"ntt 2^{args.log_ntt}: {ms:.3f} ms, {melem:.1f} Melem/s; pass {per_pass:.3f} ms ({pass_gbps:.0f} GB/s, "
-        f"{ntt_tmads:.2f} Tmad/s of {peak:.2f})")
+        f"{ntt_tmads:.2f} Tmad/s; peak {peaks['derived']:.2f} derived)")
 
     traffic = _traffic("k_ntt_pass", "ntt_log_n", args.log_ntt)
     extras = {}
@@ -376,9 +400,8 @@ def _ntt_and_pipeline(args, gb, Fr, workloads, log, quotient: bool = True) -> di
                 "kernel": "k_ntt_pass",
                 "bound": "valu",
                 "achieved": round(ntt_tmads, 4),
-                "peak": round(peak, 4),
+                **_peak_fields(ntt_tmads, peaks),
                 "unit": "Tmad/s",
-                "frac": round(ntt_tmads / peak, 4) if peak else None,
                 "work_per_unit": "one butterfly: 1 Fr mul x 136 32-bit mad; n/2 butterflies x log n stages",
                 "hbm": {"achieved": round(pass_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                         "frac": round(pass_gbps / HBM_PEAK_GBPS, 4), "bytes_per_pass": 64 * n},
@@ -395,31 +418,47 @@ def _ntt_and_pipeline(args, gb, Fr, workloads, log, quotient: bool = True) -> di
 def _batched(args, gb, points, srs, workloads, log) -> dict:
     """
     The batched form: 64 MSMs with sizes log-uniform in [2^10, 2^20] over prefixes of the
-    2^20-point basis (resident, with window tables) plus 32 forward NTTs of sizes 2^k, k uniform in {10..20}, as one graph
-    replay, against the same work through the per-call path (one zk_msm_g1 per MSM, each read
-    back before the next, as msm_pippenger_gpu returns a point; one zk_ntt per transform).
+    2^20-point basis (resident, with window tables 12 and 16) plus 32 forward NTTs of sizes 2^k,
+    k uniform in {10..20}, as one graph replay (the workload the full-size parity test checks:
+    workloads.batched_workload and its per-job seeds). Two per-call baselines run the same work
+    one call at a time, each MSM read back before the next starts (as msm_pippenger_gpu returns a
+    point) and one zk_ntt per transform:
+      raw    - zk_msm_g1 over the raw points (no precomputation),
+      tables - a one-job zk_msm_g1_batch over the SRS with its window tables, the batched form's
+               own precomputation without the batching.
     """
     from zkgpu.batch import MsmBatch, NttBatch, capture
     from zkgpu.field256_gpu import BN254_FR
     from zkgpu.field import FIELD_MODULUS as R
     from zkgpu.ntt import get_ntt_root
 
-    msm_sizes = workloads.log_uniform_sizes(64, 10, 20, 0x5EED10)
-    ntt_sizes = workloads.log_uniform_pow2(32, 10, 20, 0x5EED11)
+    import ctypes
+
+    msm_sizes, ntt_sizes = workloads.batched_workload()
     log(f"batched: 64 MSMs ({sum(msm_sizes)} points) + 32 NTTs ({sum(ntt_sizes)} elements)")
-    pool = gb.asarray(workloads.witness_rows_fast(1 << 20, 0x5EED12))
-    scal = [pool[: n] for n in msm_sizes]
-    ntt_in = [BN254_FR.from_limbs(workloads.uniform_rows(n, 0x5EED13 + i)) for i, n in enumerate(ntt_sizes)]
+    scal = [gb.asarray(workloads.batched_msm_scalars(j, n)) for j, n in enumerate(msm_sizes)]
+    ntt_in = [BN254_FR.from_limbs(workloads.batched_ntt_rows(i, n)) for i, n in enumerate(ntt_sizes)]
     ntt_out = [gb.empty((n, 4)) for n in ntt_sizes]
     outs = [gb.empty((3, 4)) for _ in msm_sizes]
     omegas = {n: gb.limbs_of(get_ntt_root(n, R)) for n in set(ntt_sizes)}
+    one = [((ctypes.c_void_p * 1)(s.ptr), (ctypes.c_int64 * 1)(n)) for s, n in zip(scal, msm_sizes)]
 
-    def plain() -> None:
+    def ntts() -> None:
+        for x, y, n in zip(ntt_in, ntt_out, ntt_sizes):
+            gb.call("zk_ntt", 0, x.ptr, y.ptr, n.bit_length() - 1, 0, omegas[n], None, gb.current_stream())
+
+    def plain_raw() -> None:
         for s, n, o in zip(scal, msm_sizes, outs):
             gb.call("zk_msm_g1", points.ptr, s.ptr, n, 0, 0, o.ptr, gb.current_stream())
             o.get()
-        for x, y, n in zip(ntt_in, ntt_out, ntt_sizes):
-            gb.call("zk_ntt", 0, x.ptr, y.ptr, n.bit_length() - 1, 0, omegas[n], None, gb.current_stream())
+        ntts()
+        gb.synchronize()
+
+    def plain_tables() -> None:
+        for (ptrs, sizes), o in zip(one, outs):
+            gb.call("zk_msm_g1_batch", srs.handle, 1, ptrs, sizes, None, o.ptr, gb.current_stream())
+            o.get()
+        ntts()
         gb.synchronize()
 
     st = gb.Stream()
@@ -432,50 +471,55 @@ def _batched(args, gb, points, srs, workloads, log) -> dict:
     def batched() -> None:
         with gb.stream(st.handle):
             graph.replay()
-            msm.out.get()  # every result back on the host, as in the per-call path
+            msm.out.get()  # every result back on the host, as in the per-call paths
         st.synchronize()
 
-    plain()
-    batched()
+    def clock(fn, reps: int) -> float:
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
     reps = max(2, args.steps // 2)
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        plain()
-    plain_ms = (time.perf_counter() - t0) * 1e3 / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        batched()
-    batch_ms = (time.perf_counter() - t0) * 1e3 / reps
-    # the batched results equal the per-call ones (as group elements)
+    raw_ms = clock(plain_raw, reps)
+    tables_ms = clock(plain_tables, reps)
+    batch_ms = clock(batched, reps)
+    # the batched results equal the per-call ones (as group elements); the full-size parity test
+    # pins every one of them to the oracle's closed form
     from zkgpu.msm_gpu import _result
 
     got = msm.results()
-    for k in (0, 31, 63):
+    for k in range(len(msm_sizes)):
         want = _result(outs[k])
         if (got[k].is_identity() != want.is_identity()) or (not want.is_identity() and got[k].affine() != want.affine()):
             raise RuntimeError(f"bench: batched MSM {k} differs from the per-call result")
-    log(f"batched: per-call {plain_ms:.2f} ms, batched graph {batch_ms:.2f} ms, speedup {plain_ms / batch_ms:.2f}x")
-    # baseline (i): the reference's public API, Python ints in (msm_pippenger_gpu), on a sample of
-    # the MSMs, scaled to the workload by point count
+    log(f"batched: per-call raw {raw_ms:.2f} ms, per-call with tables {tables_ms:.2f} ms, batched graph "
+        f"{batch_ms:.2f} ms: {raw_ms / batch_ms:.2f}x and {tables_ms / batch_ms:.2f}x")
+    # the reference's public API, Python ints in (msm_pippenger_gpu), on a sample of the MSMs,
+    # scaled to the workload by point count
     from zkgpu.field256_gpu import BN254_FQ, _limbs_to_ints
     from zkgpu.msm_gpu import msm_pippenger_gpu
 
     sample = [k for k in range(len(msm_sizes)) if msm_sizes[k] <= (1 << 16)][:6]
     coords = _limbs_to_ints(BN254_FQ.to_limbs(points[: max(msm_sizes[k] for k in sample)].reshape(-1, 4)).get())
-    pool_host = _limbs_to_ints(pool[: max(msm_sizes[k] for k in sample)].get())
+    ints = {k: _limbs_to_ints(scal[k].get()) for k in sample}
     t0 = time.perf_counter()
     for k in sample:
         m = msm_sizes[k]
-        msm_pippenger_gpu(coords[0 : 2 * m : 2], coords[1 : 2 * m : 2], pool_host[:m])
+        msm_pippenger_gpu(coords[0 : 2 * m : 2], coords[1 : 2 * m : 2], ints[k])
     ints_s = time.perf_counter() - t0
     ints_ms_total = ints_s * 1e3 * sum(msm_sizes) / sum(msm_sizes[k] for k in sample)
     log(f"batched: public API with Python ints, {len(sample)} sampled MSMs: {ints_s:.2f} s; scaled {ints_ms_total:.0f} ms")
     return {
         "batched": {
             "workload": "64 G1 MSMs log-uniform in [2^10, 2^20] over one 2^20 SRS + 32 Fr NTTs 2^10..2^20",
-            "per_call_ms": round(plain_ms, 3),
+            "per_call_raw_ms": round(raw_ms, 3),
+            "per_call_tables_ms": round(tables_ms, 3),
             "batched_graph_ms": round(batch_ms, 3),
-            "speedup": round(plain_ms / batch_ms, 3),
+            "speedup": round(tables_ms / batch_ms, 3),
+            "speedup_note": "against the per-call path over the same SRS and window tables (batching alone)",
+            "speedup_vs_raw_per_call": round(raw_ms / batch_ms, 3),
             "per_call_python_ints_ms_est": round(ints_ms_total, 1),
             "per_call_python_ints_note": f"msm_pippenger_gpu with Python ints on {len(sample)} of the MSMs "
                                          "(sizes up to 2^16), scaled by point count; the NTTs excluded",
@@ -488,9 +532,11 @@ def _batched(args, gb, points, srs, workloads, log) -> dict:
 def _served(args, gb, srs, log) -> dict:
     """
     The served form: Groth16 proof jobs (quotient at size n with coset shift 7 + three G1 MSMs
-    of size n over the shared 2^20 SRS), n = 2^k with k uniform in {12..20}. Capacity with all
-    jobs queued at once; latency under Poisson arrivals at a fixed rate; the per-call path
-    (one quotient call and three MSM calls per job, in order) timed on the first jobs.
+    of size n over the shared 2^20 SRS), n = 2^k with k uniform in {12..20}. Capacity comes from
+    all jobs queued at once; then Poisson arrivals at 0.5, 0.8 and 0.95 of that capacity give the
+    latency and batch-size curve (the headline p50/p99 at 0.8). The per-call paths (one quotient
+    call and three MSM calls per job, in order, each result read back) are timed on the first
+    jobs, over the raw points and over the SRS with its window tables.
     """
     from zkgpu.serve import ProofServer, WitnessPools, run_plain, served_workload
 
@@ -508,56 +554,208 @@ def _served(args, gb, srs, log) -> dict:
     cap = server.serve(served_workload(pools, args.serve_jobs, None, seed=0x5EED06)).summary()
     log(f"served: capacity {cap['jobs_per_s']} jobs/s in {cap['batches']} batches")
     gb.synchronize()
-    log(f"served: Poisson arrivals at {args.serve_rate} jobs/s")
-    rep = server.serve(served_workload(pools, args.serve_jobs, args.serve_rate, seed=0x5EED06)).summary()
-    log(f"served: p50 {rep['p50_ms']} ms, p99 {rep['p99_ms']} ms, {rep['jobs_per_s']} jobs/s")
-    gb.synchronize()
+    curve = []
+    fracs = [0.5, 0.8, 0.95]
+    rates = [round(f * cap["jobs_per_s"], 1) for f in fracs]
+    if args.serve_rate:
+        fracs.append(round(args.serve_rate / cap["jobs_per_s"], 3))
+        rates.append(args.serve_rate)
+    for f, rate in zip(fracs, rates):
+        rep = server.serve(served_workload(pools, args.serve_jobs, rate, seed=0x5EED06)).summary()
+        gb.synchronize()
+        curve.append({"load": f, "rate_jobs_s": rate, "p50_ms": rep["p50_ms"], "p99_ms": rep["p99_ms"],
+                      "mean_ms": rep.get("mean_ms"), "jobs_per_s": rep["jobs_per_s"],
+                      "mean_batch_jobs": rep["mean_batch_jobs"], "batches": rep["batches"]})
+        log(f"served: {rate} jobs/s ({f:.2f} of capacity): p50 {rep['p50_ms']} ms, p99 {rep['p99_ms']} ms, "
+            f"mean batch {rep['mean_batch_jobs']} jobs")
+    head = curve[1]
     sample = served_workload(pools, args.plain_sample, None, seed=0x5EED06)
-    run_plain(sample[0], srs.points)
-    gb.synchronize()
-    t0 = time.perf_counter()
-    for j in sample:
-        run_plain(j, srs.points)
-    gb.synchronize()
-    plain_s = time.perf_counter() - t0
-    plain_rate = len(sample) / plain_s
-    log(f"served: per-call path {plain_rate:.1f} jobs/s on the first {len(sample)} jobs")
+    per_call = {}
+    for name, kw in (("raw", {}), ("tables", {"srs": srs})):
+        run_plain(sample[0], srs.points, **kw)
+        gb.synchronize()
+        t0 = time.perf_counter()
+        for j in sample:
+            run_plain(j, srs.points, **kw)
+        gb.synchronize()
+        per_call[name] = len(sample) / (time.perf_counter() - t0)
+        log(f"served: per-call path ({name}) {per_call[name]:.1f} jobs/s on the first {len(sample)} jobs")
     return {
         "served": {
             "workload": f"{args.serve_jobs} Groth16 jobs, n = 2^k, k uniform in 12..20: quotient + 3 G1 MSMs over a 2^20 SRS",
-            "rate_jobs_s": args.serve_rate,
-            "p50_ms": rep["p50_ms"],
-            "p99_ms": rep["p99_ms"],
-            "poisson": rep,
+            "rate_jobs_s": head["rate_jobs_s"],
+            "p50_ms": head["p50_ms"],
+            "p99_ms": head["p99_ms"],
+            "latency_vs_load": curve,
             "capacity": cap,
-            "per_call_jobs_s": round(plain_rate, 2),
+            "per_call_tables_jobs_s": round(per_call["tables"], 2),
+            "per_call_raw_jobs_s": round(per_call["raw"], 2),
             "per_call_sample_jobs": len(sample),
-            "capacity_vs_per_call": round(cap["jobs_per_s"] / plain_rate, 3),
+            "capacity_vs_per_call": round(cap["jobs_per_s"] / per_call["tables"], 3),
+            "capacity_vs_per_call_note": "against the per-call path over the same SRS and window tables",
+            "capacity_vs_raw_per_call": round(cap["jobs_per_s"] / per_call["raw"], 3),
         }
     }
 
 
-def _cpu_baseline(points, scalars_host, gb, log) -> dict:
-    """The oracle's Python msm_pippenger on 2^18 points of the same workload, one core."""
-    sys.path.insert(0, str(ROOT / "oracle"))
-    from zkgpu.field256_gpu import BN254_FQ, _limbs_to_ints
-    from zkref.curve import Point
-    from zkref.msm import msm_pippenger
+def _host_cpu() -> tuple[str, int]:
+    """The host CPU's model name and the cores this process may run on."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
 
+
+def _cpu_ntt_worker(reps: int) -> float:
+    """Seconds for `reps` oracle NTTs of 2^12 (one worker process of the parallel-efficiency probe)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    sys.path.insert(0, str(ROOT / "zk-gpu-accelerator_amd"))
+    from zkgpu.field import FIELD_MODULUS as R
+    from zkgpu.workloads import ntt_coefficients
+    from zkref.ntt import ntt
+
+    a = ntt_coefficients(1 << 12)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ntt(a, R)
+    return time.perf_counter() - t0
+
+
+def _parallel_efficiency(procs: int, reps: int = 20) -> float:
+    """Per-process speed with `procs` oracle processes running at once, relative to one alone
+    (fresh interpreters that never touch the GPU)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+
+    ctx = mp.get_context("spawn")
+    with ProcessPoolExecutor(1, mp_context=ctx) as pool:
+        pool.submit(_cpu_ntt_worker, 1).result()
+        alone = pool.submit(_cpu_ntt_worker, reps).result()
+    with ProcessPoolExecutor(procs, mp_context=ctx) as pool:
+        list(pool.map(_cpu_ntt_worker, [1] * procs))
+        together = list(pool.map(_cpu_ntt_worker, [reps] * procs))
+    return alone / max(together)
+
+
+def _cpu_baseline(points, scalars_host, gb, log) -> dict:
+    """
+    The reference's CPU path (the oracle's Python-int restatement of ntt, Radix2Domain's
+    quotient and msm_pippenger; CPython, one core) on the host, per workload:
+      headline - msm_pippenger on the first 2^18 points and scalars of the 2^20 workload;
+      C1       - the NTT at 2^12, median of 5 after a warm-up;
+      C2       - the MSM at 2^16, single-shot, checked against the closed form;
+      C3       - the quotient pipeline at 2^22, estimated from a single-shot run at 2^16 scaled
+                 by n log n (the full run takes minutes on one core);
+      C4, C5   - estimates from cost models calibrated on the runs above (MSM seconds per
+                 Pippenger bucket addition at the reference's window, NTT seconds per n log n),
+                 on 1 core and on P worker processes with P the host cores available (at most
+                 16), scaled by the parallel efficiency measured with P oracle processes at once.
+    """
+    import statistics
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from zkgpu import workloads
+    from zkgpu.field import FIELD_MODULUS as R
+    from zkgpu.field256_gpu import BN254_FQ, _limbs_to_ints
+    from zkgpu.msm import pippenger_window_size
+    from zkref import fast
+    from zkref.curve import G, Point
+    from zkref.domain import groth16_quotient
+    from zkref.field import limbs_to_ints
+    from zkref.msm import msm_pippenger
+    from zkref.ntt import ntt
+
+    model, cores = _host_cpu()
+    procs = min(cores, 16)
+
+    def ops(n: int) -> int:  # bucket additions of the reference's Pippenger at its window
+        w = pippenger_window_size(n)
+        return -(-254 // w) * (n + (2 << w))
+
+    # headline sample
     sample = 1 << 18
     coords = _limbs_to_ints(BN254_FQ.to_limbs(points[:sample].reshape(-1, 4)).get())
     pts = [Point(coords[2 * i], coords[2 * i + 1]) for i in range(sample)]
     sc = _limbs_to_ints(scalars_host[:sample])
-    log(f"cpu baseline: oracle msm_pippenger on {sample} points (1 core)")
+    log(f"cpu baseline: oracle msm_pippenger on {sample} points (1 core, {model})")
     t0 = time.perf_counter()
     msm_pippenger(pts, sc)
-    dt = time.perf_counter() - t0
+    head_s = time.perf_counter() - t0
+    del pts, coords
+
+    # C1
+    a = workloads.ntt_coefficients(1 << 12)
+    ntt(a, R)
+    c1 = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        ntt(a, R)
+        c1.append(time.perf_counter() - t0)
+    c1_ms = statistics.median(c1) * 1e3
+
+    # C2
+    n2 = 1 << 16
+    keys2 = workloads.key_rows(n2, 0x5EED02)
+    flat = limbs_to_ints(fast.fixed_base_g1(keys2).reshape(-1, 4))
+    pts2 = [Point(flat[2 * i], flat[2 * i + 1]) for i in range(n2)]
+    sc2_rows = workloads.witness_rows(n2, 0x5EED02)
+    sc2 = limbs_to_ints(sc2_rows)
+    t0 = time.perf_counter()
+    got2 = msm_pippenger(pts2, sc2)
+    c2_s = time.perf_counter() - t0
+    e2 = fast.dot_mod_r(keys2, sc2_rows)
+    c2_ok = (got2.is_identity() and e2 == 0) or got2.affine() == (G * e2).affine()
+    del pts2, flat
+
+    # C3 at 2^16, scaled
+    n3 = 1 << 16
+    abc = [_limbs_to_ints(workloads.uniform_rows(n3, s_)) for s_ in (0x5EED31, 0x5EED32, 0x5EED33)]
+    t0 = time.perf_counter()
+    groth16_quotient(*abc, 7)
+    c3_sample_s = time.perf_counter() - t0
+    n3_full = 1 << 22
+    scale3 = (n3_full * 22) / (n3 * 16)
+    c3_est_s = c3_sample_s * scale3
+
+    # cost models
+    alpha = statistics.mean([head_s / ops(sample), c2_s / ops(n2)])  # s per bucket addition
+    beta = statistics.mean([c1_ms * 1e-3 / (4096 * 12), c3_sample_s / 7 / (n3 * 16)])  # s per n log n of one NTT
+    msm_sizes, ntt_sizes = workloads.batched_workload()
+    c4_1core_s = sum(alpha * ops(n) for n in msm_sizes) + sum(beta * n * (n.bit_length() - 1) for n in ntt_sizes)
+    job_s = [7 * beta * (1 << k) * k + 3 * alpha * ops(1 << k) for k in range(12, 21)]
+    c5_1core_jobs_s = 1.0 / statistics.mean(job_s)
+    log(f"cpu baseline: parallel efficiency with {procs} oracle processes")
+    eff = _parallel_efficiency(procs)
+    log(f"cpu baseline: headline sample {head_s:.1f} s, C1 {c1_ms:.1f} ms, C2 {c2_s:.1f} s, C3 est {c3_est_s:.0f} s, "
+        f"C4 est {c4_1core_s:.0f} s (1 core), C5 est {c5_1core_jobs_s:.4f} jobs/s (1 core), efficiency {eff:.2f}")
     return {
-        "value": round(sample / dt / 1e6, 6),
+        "value": round(sample / head_s / 1e6, 6),
         "unit": "Mpts/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"first {sample} points and scalars of the 2^20 workload, Python-int Pippenger (w={_ref_w(sample)}), {dt:.1f} s",
+        "sample": f"first {sample} points and scalars of the 2^20 workload, Python-int Pippenger (w={_ref_w(sample)}), "
+                  f"{head_s:.1f} s",
+        "host_cpu": model,
+        "host_cores": cores,
+        "configs": {
+            "C1_ntt_2p12": {"median_ms": round(c1_ms, 3), "runs": 5, "cores": 1, "kind": "port"},
+            "C2_msm_2p16": {"s": round(c2_s, 3), "mpts_s": round(n2 / c2_s / 1e6, 6), "cores": 1, "kind": "port",
+                            "single_shot": True, "verified_closed_form": bool(c2_ok)},
+            "C3_groth16_2p22": {"s_est": round(c3_est_s, 1), "cores": 1, "kind": "port", "estimate": True,
+                                "basis": f"single-shot run at 2^16 ({c3_sample_s:.2f} s) x n log n ratio {scale3:.0f}"},
+            "C4_batched": {"s_est_1core": round(c4_1core_s, 1), "s_est_P": round(c4_1core_s / (procs * eff), 1),
+                           "P": procs, "parallel_efficiency": round(eff, 3), "estimate": True},
+            "C5_served": {"jobs_s_est_1core": round(c5_1core_jobs_s, 5), "jobs_s_est_P": round(c5_1core_jobs_s * procs * eff, 4),
+                          "P": procs, "parallel_efficiency": round(eff, 3), "estimate": True},
+            "cost_model": {"msm_s_per_bucket_add": alpha, "ntt_s_per_nlogn": beta,
+                           "note": "bucket additions = ceil(254/w) (n + 2^(w+1)) at the reference's window w; "
+                                   "calibrated on the headline sample and C2 (MSM), C1 and the C3 sample (NTT)"},
+        },
     }
 
 

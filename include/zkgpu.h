@@ -169,6 +169,12 @@ int zk_g1_fixed_base(const uint64_t* k, int64_t n, uint64_t* out_affine, void* s
  * (0 = automatic); chunk_len: bucket entries per thread in the accumulation (0 = automatic). */
 int zk_msm_g1(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window,
               int chunk_len, uint64_t* out_jac, void* stream);
+/* zk_msm_g1 that also returns in *entries the number of (bucket, point) entries its bucket
+ * accumulation processed: the nonzero signed digits of the scalars' GLV halves, the unit of work
+ * a measurement prices the accumulation by. Synchronises stream. No counterpart in the reference
+ * (measurement only). */
+int zk_msm_g1_entries(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window,
+                      int chunk_len, uint64_t* out_jac, uint64_t* entries, void* stream);
 /* zk_msm_g1 with no allocation: no counterpart in the reference; for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). Same arguments, kernels and Jacobian result as zk_msm_g1; every device buffer of the

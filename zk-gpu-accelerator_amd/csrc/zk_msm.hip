@@ -1637,7 +1637,7 @@ int convert_points(const uint64_t* points, int64_t n, AffRec* out, bool with_phi
 }
 
 int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, int k0, uint64_t* out_jac,
-           hipStream_t stream) {
+           hipStream_t stream, uint64_t* entries) {
   if (n < 0 || n > 0x3fffffff) return fail("MSM size must be in [0, 2^30)");
   if (c < 0 || c > 20) return fail("window must be in 1..20");
   const bool glv = g_glv != 0;
@@ -1654,6 +1654,13 @@ int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, in
   MsmPlan* plan = nullptr;
   if (int rc = msm_plan_create(pts, {job}, k0, stream, &plan)) return rc;
   int rc = msm_plan_launch(plan, stream);
+  if (rc == 0 && entries) {  // the sorted-entry count the accumulation processed
+    uint32_t m = 0;
+    hipError_t err = hipMemcpyAsync(&m, plan->m_dev, sizeof(m), hipMemcpyDeviceToHost, stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(stream);
+    rc = err == hipSuccess ? 0 : fail_hip(err, "reading the entry count");
+    *entries = m;
+  }
   msm_plan_destroy(plan);  // blocks return to this stream's cache: later work on it is ordered after the launch
   return rc;
 }
@@ -1779,6 +1786,11 @@ int zk_msm_get_segment(int* log_seg) {
 int zk_msm_g1(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window, int chunk_len,
               uint64_t* out_jac, void* stream) {
   return msm_g1(points_affine, scalars, n, window, chunk_len, out_jac, (hipStream_t)stream);
+}
+
+int zk_msm_g1_entries(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window, int chunk_len,
+                      uint64_t* out_jac, uint64_t* entries, void* stream) {
+  return msm_g1(points_affine, scalars, n, window, chunk_len, out_jac, (hipStream_t)stream, entries);
 }
 
 int zk_msm_g1_noalloc(const uint64_t* points_affine, const uint64_t* scalars, int64_t n, int window, int chunk_len,

@@ -10,7 +10,9 @@ namespace zk {
 int default_window(int64_t n);
 
 // out_jac (device, 12 limbs, 256-form) = sum s_i P_i; c <= 0 and k0 <= 0 pick defaults.
-int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, int k0, uint64_t* out_jac, hipStream_t stream);
+// entries (optional): receives the sorted-entry count the accumulation processed (synchronises).
+int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, int k0, uint64_t* out_jac, hipStream_t stream,
+           uint64_t* entries = nullptr);
 
 // Looks up the current device's address of the no-alloc MSM's scratch array (once per device).
 int msm_scratch_init();

@@ -120,6 +120,7 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_g1_on_curve": [_vp, _i64, _i64p, _vp],
     "zk_g1_fixed_base": [_vp, _i64, _vp, _vp],
     "zk_msm_g1": [_vp, _vp, _i64, _i32, _i32, _vp, _vp],
+    "zk_msm_g1_entries": [_vp, _vp, _i64, _i32, _i32, _vp, _u64p, _vp],
     "zk_msm_g1_noalloc": [_vp, _vp, _i64, _i32, _i32, _vp, _vp],
     "zk_msm_noalloc_max_bytes": [ctypes.POINTER(_u64)],
     "zk_srs_create": [_vp, _i64, ctypes.POINTER(_vp), _vp],

@@ -358,11 +358,15 @@ def served_workload(pools: WitnessPools, count: int, rate: float | None, *, lo: 
     return [pools.job(i, k, t) for i, (k, t) in enumerate(zip(sizes, times))]
 
 
-def run_plain(job: ProofJob, srs_points: Any) -> None:
+def run_plain(job: ProofJob, srs_points: Any, *, srs: Any = None) -> None:
     """
     The per-call path for one job (the reference's structure): one quotient call and three
-    separate MSM calls, each waiting for its result before the next starts.
+    separate MSM calls, each waiting for its result before the next starts. The MSMs run over
+    the raw points (zk_msm_g1), or, given srs, as one-job batches over the SRS and its window
+    tables (zk_msm_g1_batch: the same precomputation the batched form uses, without batching).
     """
+    import ctypes
+
     from zkgpu import gpu_backend as gb
     from zkgpu.domain import Radix2Domain
 
@@ -371,6 +375,10 @@ def run_plain(job: ProofJob, srs_points: Any) -> None:
     outs = []
     for s in job.scalars:
         out = gb.empty((3, 4))
-        gb.call("zk_msm_g1", srs_points.ptr, s.ptr, job.n, 0, 0, out.ptr, gb.current_stream())
+        if srs is None:
+            gb.call("zk_msm_g1", srs_points.ptr, s.ptr, job.n, 0, 0, out.ptr, gb.current_stream())
+        else:
+            ptrs, sizes = (ctypes.c_void_p * 1)(s.ptr), (ctypes.c_int64 * 1)(job.n)
+            gb.call("zk_msm_g1_batch", srs.handle, 1, ptrs, sizes, None, out.ptr, gb.current_stream())
         outs.append(out.get())
     job.msm_jacobian = np.stack(outs)
