@@ -70,6 +70,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--ntt-only", action="store_true",
                     help="only the forward NTT at --log-ntt (for the pass kernel's trace and byte counters)")
+    ap.add_argument("--field-only", action="store_true",
+                    help="only the 2^24 Fr products (for k_binary's trace and counters)")
     ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs of the served workload")
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="an extra Poisson arrival rate (jobs/s) beside 0.5, 0.8 and 0.95 of the measured capacity")
@@ -98,6 +100,10 @@ def main() -> int:
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     if args.suite:
         _reference_suite(Path(args.suite), gb, log)
+        return 0
+
+    if args.field_only:
+        print(json.dumps(_field_mul(args, gb, BN254_FR, workloads, log)), flush=True)
         return 0
 
     if args.ntt_only:
@@ -153,6 +159,7 @@ def main() -> int:
     traffic = _traffic("k_accumulate", "msm_points", n)
     extras: dict = {}
     if not args.no_extras:
+        extras.update(_public_call(gb, points, scalars_host, ms, log))
         extras.update(_msm_2p16(args, gb, workloads, msm_g1_jacobian, log))
         extras.update(_ntt_and_pipeline(args, gb, BN254_FR, workloads, log))
         extras.update(_field_mul(args, gb, BN254_FR, workloads, log))
@@ -200,6 +207,7 @@ def main() -> int:
             "avg_launch_ms": round(acc_avg, 4),
             "units_per_launch": entries,
             "units_source": "the plan's sorted-entry count (zk_msm_g1_entries on the timed inputs)",
+            "counters_source": "profiles/k_accumulate_msm_2p20_counters.txt",
             "work_per_unit": "one mixed addition: 11 Fq mul x 136 32-bit mad; 72 bytes (key, point ref, affine point)",
         },
         "cpu_baseline": cpu,
@@ -207,6 +215,32 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _public_call(gb, points, scalars_host, resident_ms: float, log) -> dict:
+    """The reference's public call at the headline size: msm_pippenger_gpu with Python-int
+    coordinates and scalars (host conversion, PCIe upload, on-device validation, the MSM, the
+    result back as a Point), median of 3 after one warm-up; the lists are built untimed."""
+    import statistics
+
+    from zkgpu.field256_gpu import BN254_FQ, _limbs_to_ints
+    from zkgpu.msm_gpu import msm_pippenger_gpu
+
+    n = points.shape[0]
+    coords = BN254_FQ.to_ints(points.reshape(-1, 4))
+    xs, ys = coords[0::2], coords[1::2]
+    sc = _limbs_to_ints(scalars_host)
+    msm_pippenger_gpu(xs, ys, sc)
+    samples = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        msm_pippenger_gpu(xs, ys, sc)
+        samples.append((time.perf_counter() - t0) * 1e3)
+    ms = statistics.median(samples)
+    log(f"public call, 2^{n.bit_length() - 1} Python ints: {ms:.1f} ms ({ms / resident_ms:.0f}x the device-resident step)")
+    return {"msm_public_python_ints": {"n": n, "ms": round(ms, 2), "vs_resident": round(ms / resident_ms, 1),
+                                       "note": "msm_pippenger_gpu(xs, ys, scalars) with Python ints: host conversion, "
+                                               "PCIe upload, on-device validation, MSM, result as a Point"}}
 
 
 def _msm_2p16(args, gb, workloads, msm_g1_jacobian, log) -> dict:
@@ -330,7 +364,9 @@ def _field_mul(args, gb, Fr, workloads, log) -> dict:
             "gmul_s": round(gmul, 3),
             "roofline": {"kernel": "k_binary<mul>", "bound": "hbm", "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS,
                          "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4),
-                         "algorithmic_bytes_per_launch": 96 * n, "valu_tmad_s": round(tmads, 3)},
+                         "algorithmic_bytes_per_launch": 96 * n, "valu_tmad_s": round(tmads, 3),
+                         "avg_launch_ms": round(ms, 4),
+                         "trace_and_counters_source": "profiles/k_binary_fr_mul_2p24_counters.txt"},
         }
     }
 
@@ -409,6 +445,7 @@ def _ntt_and_pipeline(args, gb, Fr, workloads, log, quotient: bool = True) -> di
                 "traffic_source": traffic.get("source") if traffic else None,
                 "algorithmic_bytes_per_launch": 64 * n,
                 "avg_launch_ms": round(per_pass, 4),
+                "counters_source": "profiles/k_ntt_pass_fr_2p22_counters.txt",
             },
         },
         **extras,

@@ -167,3 +167,28 @@ def test_bulk_int_rows_match_python_conversion():
     with pytest.raises(TypeError, match=r"^'str' object cannot be interpreted as an integer"):
         fg._rows_from_ints([1, "2"], r)
     assert fg._rows_from_ints([], r)[0].shape == (0, 4)
+
+
+def test_bulk_conversion_on_worker_threads_matches_the_serial_one():
+    """Long inputs are converted on several threads: the rows equal the serial conversion's for
+    every kind of element wherever it sits, and the first bad element in index order raises."""
+    import random
+
+    import numpy as np
+
+    from zkgpu import field256_gpu as fg
+
+    r = fg.BN254_SCALAR_MODULUS
+    rng = random.Random(11)
+    n = (1 << 18) + 5
+    vals = [rng.getrandbits(rng.choice((1, 64, 254, 256))) for _ in range(n)]
+    odd = [-1, -(2**300), 2**256, 2**400 + 3, True, False, np.int64(-9), np.uint64(2**63 + 1)]
+    for pos in rng.sample(range(n), 200):
+        vals[pos] = rng.choice(odd)
+    rows, _ = fg._rows_from_ints(vals, r, "scalars")
+    want = [v if 0 <= v < 2**256 else v % r for v in (int(x) for x in vals)]
+    assert fg._limbs_to_ints(rows) == want
+    bad = list(vals)
+    bad[n - 3], bad[n // 2] = "late", 1.5
+    with pytest.raises(TypeError, match="'float' object"):
+        fg._rows_from_ints(bad, r, "scalars")
