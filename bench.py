@@ -72,10 +72,12 @@ def main() -> int:
                     help="only the forward NTT at --log-ntt (for the pass kernel's trace and byte counters)")
     ap.add_argument("--field-only", action="store_true",
                     help="only the 2^24 Fr products (for k_binary's trace and counters)")
+    ap.add_argument("--ntt64-only", action="store_true",
+                    help="only the 64-bit NTT at 2^23 (for k_ntt64_pass's trace and byte counters)")
     ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs of the served workload")
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="an extra Poisson arrival rate (jobs/s) beside 0.5, 0.8 and 0.95 of the measured capacity")
-    ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="HBM budget of the server")
+    ap.add_argument("--serve-budget-gib", type=float, default=128.0, help="HBM budget of the server")
     ap.add_argument("--plain-sample", type=int, default=200, help="jobs timed through the per-call path")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="copy 256 MiB with zk_copy first (the known-size kernel for scaling byte counters)")
@@ -100,6 +102,12 @@ def main() -> int:
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     if args.suite:
         _reference_suite(Path(args.suite), gb, log)
+        return 0
+
+    if args.ntt64_only:
+        if args.traffic_probe:
+            _copy_probe(gb)
+        print(json.dumps(_ntt64(args, gb, log)), flush=True)
         return 0
 
     if args.field_only:
@@ -163,6 +171,7 @@ def main() -> int:
         extras.update(_msm_2p16(args, gb, workloads, msm_g1_jacobian, log))
         extras.update(_ntt_and_pipeline(args, gb, BN254_FR, workloads, log))
         extras.update(_field_mul(args, gb, BN254_FR, workloads, log))
+        extras.update(_ntt64(args, gb, log))
         from zkgpu.batch import SRS
 
         log("srs: device-resident basis with window tables of widths 12 and 16 (setup, not timed)")
@@ -369,6 +378,35 @@ def _field_mul(args, gb, Fr, workloads, log) -> dict:
                          "trace_and_counters_source": "profiles/k_binary_fr_mul_2p24_counters.txt"},
         }
     }
+
+
+def _ntt64(args, gb, log, log_n: int = 23) -> dict:
+    """The 64-bit NTT (the reference's ntt_gpu, int64 modulo 998244353) at 2^23 elements, its largest
+    size for that modulus, device
+    resident. Each pass reads and writes every element once (16 bytes per element per pass, the
+    algorithmic bytes of a launch); pass times from the library's kernel timers."""
+    from zkgpu.ntt_gpu import ntt_gpu
+
+    mod = 998_244_353
+    n = 1 << log_n
+    x = gb.asarray(np.random.default_rng(0x5EED64).integers(0, mod, n, dtype=np.int64))
+    ntt_gpu(x, mod)
+    ms = timed(lambda: ntt_gpu(x, mod), args.steps, 10, gb)
+    gb.prof_enable(True)
+    ntt_gpu(x, mod)
+    gb.synchronize()
+    gb.prof_read("k_ntt64_pass", reset=True)
+    timed(lambda: ntt_gpu(x, mod), args.steps, 0, gb)
+    pass_ms, passes = gb.prof_read("k_ntt64_pass", reset=True)
+    gb.prof_enable(False)
+    per_pass = pass_ms / max(passes, 1)
+    gbps = 16.0 * n / (per_pass * 1e-3) / 1e9
+    log(f"ntt64 2^{log_n}: {ms:.3f} ms ({n / (ms * 1e-3) / 1e6:.0f} Melem/s), pass {per_pass:.4f} ms, {gbps:.0f} GB/s")
+    return {"ntt64": {"log_n": log_n, "mod": mod, "ms": round(ms, 4), "melem_s": round(n / (ms * 1e-3) / 1e6, 1),
+                      "roofline": {"kernel": "k_ntt64_pass", "bound": "hbm", "achieved": round(gbps, 1),
+                                   "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                                   "algorithmic_bytes_per_launch": 16 * n, "avg_launch_ms": round(per_pass, 4),
+                                   "passes": max(1, round(passes / args.steps))}}}
 
 
 def _ntt_and_pipeline(args, gb, Fr, workloads, log, quotient: bool = True) -> dict:

@@ -266,7 +266,7 @@ int zk_f64_pow(const void* base, int dtype, const int64_t* strides, int ndim, co
  * invertibility check of gpu_batch_pow for negative exponents (src/zkgpu/field_gpu.py:221-223). */
 int zk_f64_any_zero(const void* base, int dtype, const int64_t* strides, int ndim, const int64_t* shape,
                     uint64_t mod, int* result, void* stream);
-/* The NTT X_k = sum_j in_j omega^(jk) of 2^log_n values (log_n <= 33) modulo an odd prime
+/* The NTT X_k = sum_j in_j omega^(jk) of 2^log_n values (log_n <= 30) modulo an odd prime
  * mod < 2^63, omega a primitive 2^log_n-th root of unity; invert uses omega^-1 and scales by
  * n^-1. in (contiguous, any integer dtype, reduced on load) is left unchanged; out (contiguous
  * int64, natural order) must be another buffer. Twiddle tables are cached per (mod, omega,

@@ -311,7 +311,7 @@ def _evaluations(a: np.ndarray, mod: int, omega: int, ks: list[int]) -> list[int
 @pytest.mark.gpu
 @pytest.mark.parametrize("log_n", [20, 22, 23])
 def test_ntt_at_size_through_evaluations_and_its_inverse(log_n):
-    """2^20 and 2^22 are two passes, 2^23 three: sampled outputs equal direct evaluations of the
+    """2^20 is two passes, 2^22 and 2^23 three: sampled outputs equal direct evaluations of the
     polynomial and the inverse transform returns the input exactly."""
     n, mod = 1 << log_n, NTT_MOD
     rng = np.random.default_rng(log_n)

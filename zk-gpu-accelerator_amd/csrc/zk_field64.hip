@@ -4,7 +4,7 @@
 //
 // Products use Barrett reduction (HAC 14.42 with base 2 and k = bit length of m) on the exact
 // 128-bit product, so every modulus below 2^63 takes the same path. The NTT is a multi-pass
-// Cooley-Tukey decomposition N = n_1 n_2 ... (at most 2^11 points per pass): each pass loads a
+// Cooley-Tukey decomposition N = n_1 n_2 ... (at most 2^10 points per pass): each pass loads a
 // tile of several sub-transforms into LDS, runs their radix-2 stages there with precomputed
 // Shoup twiddles, applies the inter-pass twiddle while storing, and the last pass writes natural
 // order (and n^-1 for an inverse). A 2^20 transform is two passes, one HBM round trip each.
@@ -197,11 +197,13 @@ struct Ntt64Pass {
   uint32_t first, last, do_scale;
 };
 
-constexpr int kNttThreads = 256;
-constexpr int kNttMaxLog = 11;    // points per sub-transform
-constexpr int kNttTileLog = 13;   // points per workgroup tile (64 KiB of LDS)
-constexpr int kNtt64MaxLogN = 33;  // three passes
-// LDS of the largest tile: 2^13 points, one padding word per row, 2^10 twiddle pairs.
+constexpr int kNttThreads = 512;
+// Points per sub-transform: 2^10 keeps every tile at least 8 columns wide (64-byte runs) and its
+// LDS at 74 KiB, two workgroups per CU.
+constexpr int kNttMaxLog = 10;
+constexpr int kNttTileLog = 13;    // points per workgroup tile (64 KiB of LDS)
+constexpr int kNtt64MaxLogN = 30;  // three passes
+// LDS of the largest tile: 2^13 points, one padding word per row, the stage twiddle pairs.
 constexpr int kNttMaxLds = ((1 << kNttTileLog) + (1 << kNttTileLog) / 2 + (1 << kNttMaxLog)) * 8;
 
 __global__ void __launch_bounds__(kNttThreads) k_ntt64_pass(Ntt64Pass d) {
@@ -433,7 +435,7 @@ int zk_f64_any_zero(const void* base, int dtype, const int64_t* strides, int ndi
 
 int zk_ntt64(const void* in, int dtype, int64_t* out, int log_n, uint64_t mod, uint64_t omega, int invert, void* stream) {
   if (mod < 3 || mod >= (1ull << 63) || (mod & 1) == 0) return fail("zk_ntt64: the modulus must be an odd prime below 2^63");
-  if (log_n < 0 || log_n > kNtt64MaxLogN) return fail("zk_ntt64: log_n must be in [0, 33]");
+  if (log_n < 0 || log_n > kNtt64MaxLogN) return fail("zk_ntt64: log_n must be in [0, 30]");
   if (dtype < ZK_I8 || dtype > ZK_U64) return fail("unknown integer dtype code");
   if (in == (const void*)out) return fail("zk_ntt64: out must not be the input");
   if (omega == 0 || omega >= mod) return fail("zk_ntt64: omega must be a nonzero residue");

@@ -7,7 +7,7 @@ zkgpu.ntt.get_ntt_root (the CPU NTT's, so both transforms agree element for elem
 any values reduced mod *mod* first, the input left unchanged, and the CPU NTT's validation
 errors.
 
-The transform is zk_ntt64 in libzkgpu_hip.so: a multi-pass decomposition with at most 2^11
+The transform is zk_ntt64 in libzkgpu_hip.so: a multi-pass decomposition with at most 2^10
 points per pass, each pass one HBM round trip through LDS (two passes at 2^20, where the
 reference launches five CuPy kernels per stage). Its twiddle tables are built once per
 (size, modulus, direction) and cached on the device with the 256-bit NTT's tables;
