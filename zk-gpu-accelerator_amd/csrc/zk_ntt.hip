@@ -39,6 +39,7 @@
 #include <list>
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -49,8 +50,17 @@
 
 namespace zk {
 
-constexpr int kTile = 1024;  // rows per workgroup tile
-constexpr int kThreads = 256;
+// Rows per workgroup tile (2^kTileLog, staged in LDS as nine limb planes) and threads per
+// workgroup; a pass fuses at most kTileLog stages.
+#ifndef ZK_NTT_TILE_LOG
+#define ZK_NTT_TILE_LOG 10
+#endif
+#ifndef ZK_NTT_THREADS
+#define ZK_NTT_THREADS 256
+#endif
+constexpr int kTileLog = ZK_NTT_TILE_LOG;
+constexpr int kTile = 1 << kTileLog;
+constexpr int kThreads = ZK_NTT_THREADS;
 
 __device__ __forceinline__ void lds_put(uint32_t* lds, int slot, const Fe& x) {
 #pragma unroll
@@ -461,7 +471,7 @@ int g_fusion[29] = {0};
 std::vector<PassPlan> plan_passes(int log_n) {
   std::vector<PassPlan> out;
   const int max_k = g_fusion[log_n] > 0 ? g_fusion[log_n] : 8;
-  if (log_n <= 10 && max_k >= log_n) {
+  if (log_n <= kTileLog && max_k >= log_n) {
     out.push_back({0, log_n, 0});
     return out;
   }
@@ -470,7 +480,7 @@ std::vector<PassPlan> plan_passes(int log_n) {
   for (int p = 0; p < passes; ++p) {
     int k = (log_n - s0) / (passes - p);
     if ((log_n - s0) % (passes - p)) ++k;
-    int log_c = 10 - k;
+    int log_c = kTileLog - k;
     if (log_c > log_n - k) log_c = log_n - k;  // no more groups than exist
     out.push_back({s0, k, log_c});
     s0 += k;
@@ -736,7 +746,8 @@ int zk_ntt_cache_stats(uint64_t* tables, uint64_t* bytes, uint64_t* pinned) { re
 
 int zk_ntt_set_fusion(int log_n, int max_stages) {
   if (log_n < 1 || log_n > 28) return fail("log_n must be in [1, 28]");
-  if (max_stages < 0 || max_stages > 10) return fail("stages per pass must be in 1..10 (0 = default)");
+  if (max_stages < 0 || max_stages > kTileLog)
+    return fail("stages per pass must be in 1.." + std::to_string(kTileLog) + " (0 = default)");
   g_fusion[log_n] = max_stages;
   return 0;
 }
