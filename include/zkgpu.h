@@ -269,8 +269,8 @@ int zk_f64_any_zero(const void* base, int dtype, const int64_t* strides, int ndi
 /* The NTT X_k = sum_j in_j omega^(jk) of 2^log_n values (log_n <= 30) modulo an odd prime
  * mod < 2^63, omega a primitive 2^log_n-th root of unity; invert uses omega^-1 and scales by
  * n^-1. in (contiguous, any integer dtype, reduced on load) is left unchanged; out (contiguous
- * int64, natural order) must be another buffer. Twiddle tables are cached per (mod, omega,
- * log_n, invert) with the other NTT tables. Replaces ntt_gpu's per-stage CuPy butterflies,
+ * int64, natural order) must be another buffer. Moduli below 2^31 run on 32-bit arithmetic.
+ * Twiddle tables are cached per (mod, omega, log_n, invert) with the other NTT tables. Replaces ntt_gpu's per-stage CuPy butterflies,
  * bit-reversal gather and cached twiddle table (src/zkgpu/ntt_gpu.py:48-155). */
 int zk_ntt64(const void* in, int dtype, int64_t* out, int log_n, uint64_t mod, uint64_t omega, int invert,
              void* stream);

@@ -66,6 +66,20 @@ __device__ inline uint64_t shoup(uint64_t a, uint64_t w, uint64_t wq, uint64_t m
   return r >= m ? r - m : r;
 }
 
+// The same product for m < 2^31 in 32-bit words: wq = floor(w 2^32 / m), any a < 2^32, w < m.
+__device__ inline uint32_t shoup(uint32_t a, uint32_t w, uint32_t wq, uint32_t m) {
+  const uint32_t q = __umulhi(a, wq);
+  const uint32_t r = a * w - q * m;  // the exact value lies in [0, 2m)
+  return r >= m ? r - m : r;
+}
+
+__device__ inline uint32_t addmod(uint32_t a, uint32_t b, uint32_t m) {  // a, b < m < 2^31
+  const uint32_t s = a + b;
+  return s >= m ? s - m : s;
+}
+
+__device__ inline uint32_t submod(uint32_t a, uint32_t b, uint32_t m) { return a >= b ? a - b : a + (m - b); }
+
 // Floor-mod of an int64 into [0, m): in-range values skip the (software) 64-bit division.
 __device__ inline uint64_t reduce_i64(int64_t x, uint64_t m) {
   if (x >= 0) {
@@ -206,31 +220,36 @@ constexpr int kNtt64MaxLogN = 30;  // three passes
 // LDS of the largest tile: 2^13 points, one padding word per row, the stage twiddle pairs.
 constexpr int kNttMaxLds = ((1 << kNttTileLog) + (1 << kNttTileLog) / 2 + (1 << kNttMaxLog)) * 8;
 
+// T = uint64_t for any modulus below 2^63; T = uint32_t for moduli below 2^31 (the pass then holds
+// 32-bit residues and twiddle quotients floor(w 2^32 / m), so a Shoup product is three 32-bit
+// instructions instead of about sixteen).
+template <class T>
 __global__ void __launch_bounds__(kNttThreads) k_ntt64_pass(Ntt64Pass d) {
-  extern __shared__ uint64_t lds[];
+  extern __shared__ uint64_t lds_words[];
+  T* lds = reinterpret_cast<T*>(lds_words);
   const uint32_t R = 1u << d.log_r, CJ = 1u << d.log_cj, CR = 1u << d.log_cr;
-  const uint32_t slab = (R << d.log_cj) + 1;  // words per row of the tile, one of them padding
-  uint64_t* tw = lds + (size_t)CR * slab;    // this pass's stage twiddles, R / 2 pairs
+  const uint32_t slab = (R << d.log_cj) + 1;  // elements per row of the tile, one of them padding
+  T* tw = lds + (size_t)CR * slab;           // this pass's stage twiddles, R / 2 pairs
   const uint64_t P = 1ull << d.log_p;
   const uint64_t blk = blockIdx.x;
   const uint32_t jt_bits = d.log_s - d.log_cj;
   const uint64_t J0 = (blk & ((1ull << jt_bits) - 1)) << d.log_cj;
   const uint64_t r0 = (blk >> jt_bits) << d.log_cr;
   const uint32_t E = R << (d.log_cj + d.log_cr);
-  const uint64_t m = d.m;
+  const T m = (T)d.m;
 
   for (uint32_t i = threadIdx.x; i < R / 2; i += kNttThreads) {
     const uint64_t src = (uint64_t)i << (d.log_rmax - d.log_r);
-    tw[2 * i] = d.stage[2 * src];
-    tw[2 * i + 1] = d.stage[2 * src + 1];
+    tw[2 * i] = (T)d.stage[2 * src];
+    tw[2 * i + 1] = (T)d.stage[2 * src + 1];
   }
   // Load: columns fastest, then points, then rows; point j lands at its bit-reversed slot.
   for (uint32_t e = threadIdx.x; e < E; e += kNttThreads) {
     const uint32_t cj = e & (CJ - 1), j = (e >> d.log_cj) & (R - 1), cr = e >> (d.log_cj + d.log_r);
     const uint64_t addr = ((r0 + cr) << (d.log_r + d.log_s)) + ((uint64_t)j << d.log_s) + J0 + cj;
-    const uint64_t v = d.first ? reduce_i64(load_i64(d.in, d.in_dtype, addr), m) : (uint64_t)((const int64_t*)d.in)[addr];
+    const uint64_t v = d.first ? reduce_i64(load_i64(d.in, d.in_dtype, addr), d.m) : (uint64_t)((const int64_t*)d.in)[addr];
     const uint32_t jr = d.log_r ? (__brev(j) >> (32 - d.log_r)) : 0;
-    lds[cr * slab + (jr << d.log_cj) + cj] = v;
+    lds[cr * slab + (jr << d.log_cj) + cj] = (T)v;
   }
   __syncthreads();
   // Radix-2 decimation-in-time stages on bit-reversed input give natural order.
@@ -241,9 +260,9 @@ __global__ void __launch_bounds__(kNttThreads) k_ntt64_pass(Ntt64Pass d) {
       const uint32_t tt = t & (h - 1);
       const uint32_t i = ((t >> lh) << (lh + 1)) + tt;
       const uint32_t wi = tt << (d.log_r - 1 - lh);  // w_(2h)^tt = w_R^(tt R / 2h)
-      uint64_t* row = lds + cr * slab + cj;
-      const uint64_t u = row[i << d.log_cj], v = row[(i + h) << d.log_cj];
-      const uint64_t x = shoup(v, tw[2 * wi], tw[2 * wi + 1], m);
+      T* row = lds + cr * slab + cj;
+      const T u = row[i << d.log_cj], v = row[(i + h) << d.log_cj];
+      const T x = shoup(v, tw[2 * wi], tw[2 * wi + 1], m);
       row[i << d.log_cj] = addmod(u, x, m);
       row[(i + h) << d.log_cj] = submod(u, x, m);
     }
@@ -261,14 +280,14 @@ __global__ void __launch_bounds__(kNttThreads) k_ntt64_pass(Ntt64Pass d) {
       cr = e & (CR - 1);
       k = (e >> d.log_cr) & (R - 1);
     }
-    uint64_t v = lds[cr * slab + (k << d.log_cj) + cj];
+    T v = lds[cr * slab + (k << d.log_cj) + cj];
     if (!d.last) {
       const uint64_t ex = P * (J0 + cj) * k;  // < N
       const uint64_t eh = ex >> d.log_t, el = ex & ((1ull << d.log_t) - 1);
-      v = shoup(v, d.hi[2 * eh], d.hi[2 * eh + 1], m);
-      v = shoup(v, d.lo[2 * el], d.lo[2 * el + 1], m);
+      v = shoup(v, (T)d.hi[2 * eh], (T)d.hi[2 * eh + 1], m);
+      v = shoup(v, (T)d.lo[2 * el], (T)d.lo[2 * el + 1], m);
     } else if (d.do_scale) {
-      v = shoup(v, d.scale, d.scale_q, m);
+      v = shoup(v, (T)d.scale, (T)d.scale_q, m);
     }
     d.out[((r0 + cr + P * k) << d.log_s) + J0 + cj] = (int64_t)v;
   }
@@ -449,14 +468,16 @@ int zk_ntt64(const void* in, int dtype, int64_t* out, int log_n, uint64_t mod, u
   const uint64_t rmax_half = log_rmax ? (1ull << (log_rmax - 1)) : 1;
   const uint64_t nlo = 1ull << log_t, nhi = 1ull << (log_n - log_t);
   const size_t words = 2 * (rmax_half + nlo + nhi);
-  const uint64_t tag[4] = {mod, omega, (uint64_t)(invert ? 1 : 0), 0x64};
+  const bool small = mod < (1ull << 31);  // 32-bit residues and Shoup quotients
+  const uint64_t tag[4] = {mod, omega, (uint64_t)(invert ? 1 : 0), small ? 0x32u : 0x64u};
+  auto quotient = [&](uint64_t v) { return small ? (uint64_t)(((u128)v << 32) / mod) : shoup_q(v, mod); };
   TableHold hold;
   const uint64_t* tab = nullptr;
   int rc = host_table(log_n, tag, words, [&](uint64_t* h) {
     const uint64_t w = invert ? powmod_host(omega, mod - 2, k) : omega;
     auto put = [&](uint64_t* dst, uint64_t v) {
       dst[0] = v;
-      dst[1] = shoup_q(v, mod);
+      dst[1] = quotient(v);
     };
     const uint64_t wr = powmod_host(w, 1ull << (log_n - log_rmax), k);  // w_Rmax
     uint64_t acc = 1;
@@ -481,8 +502,11 @@ int zk_ntt64(const void* in, int dtype, int64_t* out, int log_n, uint64_t mod, u
   }
   const int np = (int)passes.size();
   static const hipError_t attr =
-      hipFuncSetAttribute((const void*)k_ntt64_pass, hipFuncAttributeMaxDynamicSharedMemorySize, kNttMaxLds);
+      hipFuncSetAttribute((const void*)k_ntt64_pass<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, kNttMaxLds);
   ZK_HIP(attr);
+  static const hipError_t attr32 =
+      hipFuncSetAttribute((const void*)k_ntt64_pass<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, kNttMaxLds);
+  ZK_HIP(attr32);
   const void* src = in;
   for (int p = 0; p < np; ++p) {
     const PassShape& s = passes[p];
@@ -495,7 +519,7 @@ int zk_ntt64(const void* in, int dtype, int64_t* out, int log_n, uint64_t mod, u
     d.hi = hi;
     d.m = mod;
     d.scale = ninv;
-    d.scale_q = shoup_q(ninv, mod);
+    d.scale_q = quotient(ninv);
     d.in_dtype = p == 0 ? dtype : ZK_I64;
     d.log_r = s.log_r;
     d.log_rmax = log_rmax;
@@ -509,9 +533,11 @@ int zk_ntt64(const void* in, int dtype, int64_t* out, int log_n, uint64_t mod, u
     d.do_scale = invert && p == np - 1;
     const uint64_t tile = (1ull << (s.log_r + s.log_cj + s.log_cr));
     const unsigned blocks = (unsigned)(n / tile);
-    const size_t lds = ((size_t)(1u << s.log_cr) * (((size_t)1 << (s.log_r + s.log_cj)) + 1) + (1u << s.log_r)) * 8;
+    const size_t lds = ((size_t)(1u << s.log_cr) * (((size_t)1 << (s.log_r + s.log_cj)) + 1) + (1u << s.log_r)) *
+                       (small ? 4 : 8);
     if (lds > (size_t)kNttMaxLds) return fail("zk_ntt64: pass tile exceeds its LDS budget");
-    launch_timed("k_ntt64_pass", k_ntt64_pass, dim3(blocks), dim3(kNttThreads), (uint32_t)lds, st, d);
+    if (small) launch_timed("k_ntt64_pass", k_ntt64_pass<uint32_t>, dim3(blocks), dim3(kNttThreads), (uint32_t)lds, st, d);
+    else launch_timed("k_ntt64_pass", k_ntt64_pass<uint64_t>, dim3(blocks), dim3(kNttThreads), (uint32_t)lds, st, d);
     ZK_LAUNCH_CHECK("64-bit NTT pass launch");
     src = dst;
   }
