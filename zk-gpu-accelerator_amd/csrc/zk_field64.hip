@@ -252,8 +252,36 @@ __global__ void __launch_bounds__(kNttThreads) k_ntt64_pass(Ntt64Pass d) {
     lds[cr * slab + (jr << d.log_cj) + cj] = (T)v;
   }
   __syncthreads();
-  // Radix-2 decimation-in-time stages on bit-reversed input give natural order.
-  for (uint32_t lh = 0; lh < d.log_r; ++lh) {
+  // Decimation-in-time stages on bit-reversed input give natural order. Stages go in pairs
+  // (radix 4: a thread takes points i, i + h, i + 2h, i + 3h through stages lh and lh + 1 in
+  // registers, one LDS round trip and one barrier for two stages); an odd count ends radix 2.
+  uint32_t lh = 0;
+  for (; lh + 1 < d.log_r; lh += 2) {
+    const uint32_t h = 1u << lh;
+    for (uint32_t q = threadIdx.x; q < E / 4; q += kNttThreads) {
+      const uint32_t cj = q & (CJ - 1), t = (q >> d.log_cj) & (R / 4 - 1), cr = q >> (d.log_cj + d.log_r - 2);
+      const uint32_t tt = t & (h - 1);
+      const uint32_t i = ((t >> lh) << (lh + 2)) + tt;
+      const uint32_t w1 = tt << (d.log_r - 1 - lh);        // w_(2h)^tt
+      const uint32_t w2 = tt << (d.log_r - 2 - lh);        // w_(4h)^tt
+      const uint32_t w3 = (tt + h) << (d.log_r - 2 - lh);  // w_(4h)^(tt + h)
+      T* row = lds + cr * slab + cj;
+      const T a0 = row[i << d.log_cj], a1 = row[(i + h) << d.log_cj], a2 = row[(i + 2 * h) << d.log_cj],
+              a3 = row[(i + 3 * h) << d.log_cj];
+      T x = shoup(a1, tw[2 * w1], tw[2 * w1 + 1], m);
+      const T b0 = addmod(a0, x, m), b1 = submod(a0, x, m);
+      x = shoup(a3, tw[2 * w1], tw[2 * w1 + 1], m);
+      const T b2 = addmod(a2, x, m), b3 = submod(a2, x, m);
+      x = shoup(b2, tw[2 * w2], tw[2 * w2 + 1], m);
+      row[i << d.log_cj] = addmod(b0, x, m);
+      row[(i + 2 * h) << d.log_cj] = submod(b0, x, m);
+      x = shoup(b3, tw[2 * w3], tw[2 * w3 + 1], m);
+      row[(i + h) << d.log_cj] = addmod(b1, x, m);
+      row[(i + 3 * h) << d.log_cj] = submod(b1, x, m);
+    }
+    __syncthreads();
+  }
+  for (; lh < d.log_r; ++lh) {
     const uint32_t h = 1u << lh;
     for (uint32_t q = threadIdx.x; q < E / 2; q += kNttThreads) {
       const uint32_t cj = q & (CJ - 1), t = (q >> d.log_cj) & (R / 2 - 1), cr = q >> (d.log_cj + d.log_r - 1);
