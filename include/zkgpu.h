@@ -47,6 +47,14 @@ int zk_memcpy_h2d(void* dst, const void* host_src, uint64_t bytes, void* stream)
 int zk_memcpy_d2h(void* host_dst, const void* src, uint64_t bytes, void* stream);  /* synchronises stream */
 int zk_memcpy_d2d(void* dst, const void* src, uint64_t bytes, void* stream);
 int zk_memset(void* dst, int value, uint64_t bytes, void* stream);
+/* Page-locked host memory for uploads that run at PCIe speed (the public int-taking calls stage
+ * converted rows in it): zk_host_alloc returns a block of at least `bytes`, zk_host_free takes it
+ * back into a small per-process cache (blocks are pinned once and reused).
+ * zk_memcpy_h2d_async copies from such a block without waiting: the block must stay unchanged
+ * until the stream has passed the copy. No counterpart in the reference (CuPy's pageable copies). */
+int zk_host_alloc(void** ptr, uint64_t bytes);
+int zk_host_free(void* ptr);
+int zk_memcpy_h2d_async(void* dst, const void* pinned_src, uint64_t bytes, void* stream);
 /* A plain copy kernel (each byte read and written once; 16-byte aligned buffers and sizes): the
  * known-size reference that hardware byte counters are scaled by. No counterpart in the reference. */
 int zk_copy(void* dst, const void* src, uint64_t bytes, void* stream);
@@ -157,6 +165,10 @@ int zk_g1_double(const uint64_t* a, uint64_t* out, int64_t n, void* stream);
 /* Replaces the host-side Point(x, y) validation in msm_pippenger_gpu (msm_gpu.py:441-442):
  * *first_bad (host) = smallest i whose affine mont point is off y^2 = x^3 + 3, or -1. Synchronises. */
 int zk_g1_on_curve(const uint64_t* points_affine, int64_t n, int64_t* first_bad, void* stream);
+/* The same check without waiting: the first bad index, or n when every point is on the curve, is
+ * written to the device word *first_bad_dev in stream order (so a caller can go on converting
+ * other inputs and read the verdict later). */
+int zk_g1_on_curve_async(const uint64_t* points_affine, int64_t n, uint64_t* first_bad_dev, void* stream);
 /* out_affine[i] = k[i] * G for plain scalars k[i] < 2^256 (all-zero rows for the identity);
  * the SRS generator behind the device-resident basis (no counterpart: the reference builds
  * its bases with Python scalar multiplications, benchmarks/bench_msm.py:40-44). */

@@ -192,3 +192,33 @@ def test_bulk_conversion_on_worker_threads_matches_the_serial_one():
     bad[n - 3], bad[n // 2] = "late", 1.5
     with pytest.raises(TypeError, match="'float' object"):
         fg._rows_from_ints(bad, r, "scalars")
+
+
+def test_rows_into_writes_strided_rows_and_parts():
+    """zkgpu._ingest.rows_into (the public MSM's staging conversion) writes row i at 32-byte row
+    offset + i * stride of the destination, converts [start, stop) when given, leaves every other
+    row alone, and refuses a destination that is too small."""
+    import random
+
+    from zkgpu import field256_gpu as fg
+
+    r = fg.BN254_SCALAR_MODULUS
+    rng = random.Random(5)
+    vals = [rng.getrandbits(rng.choice((1, 30, 31, 60, 240, 241, 255, 256))) for _ in range(3000)]
+    vals[7], vals[9], vals[11] = -3, 2**256 + 1, 2**270
+    want = [(v if 0 <= v < 2**256 else v % r).to_bytes(32, "little") for v in vals]
+    dest = bytearray(32 * (2 * len(vals) + 3))
+    count, items = fg._ingest.rows_into(vals, r, "points_y", dest, 2, 3)
+    assert count == len(vals) and items is vals
+    assert bytes(dest[:96]) == bytes(96)
+    for i, w in enumerate(want):
+        assert dest[32 * (3 + 2 * i) : 32 * (4 + 2 * i)] == w
+        assert dest[32 * (4 + 2 * i) : 32 * (5 + 2 * i)] == bytes(32)
+    parts = bytearray(32 * len(vals))
+    for lo in range(0, len(vals), 700):
+        fg._ingest.rows_into(vals, r, "scalars", parts, 1, 0, lo, lo + 700)
+    assert bytes(parts) == b"".join(want)
+    with pytest.raises(ValueError, match="too small"):
+        fg._ingest.rows_into(vals, r, "scalars", bytearray(32 * len(vals)), 2, 0)
+    with pytest.raises(TypeError, match=r"^points_x must contain integers: 'float' object"):
+        fg._ingest.rows_into([1, 2.5], r, "points_x", bytearray(64), 1, 0)

@@ -327,3 +327,34 @@ def test_resident_points_off_the_curve_are_refused():
     pts = BN254_FQ.from_limbs(bad.reshape(-1, 4)).reshape(3, 2, 4)  # (0, 0) is not on the curve
     with pytest.raises(ValueError, match=r"\(0, 0\) is not on the BN254 curve"):
         msm_pippenger_gpu(pts, None, gb.asarray(np.ones((3, 4), dtype=np.uint64)))
+
+
+def test_public_int_path_checks_in_the_reference_order():
+    # integer types of x, then y, then the scalars, then the lengths (msm_gpu.py:432-439)
+    with pytest.raises(TypeError, match="points_x must contain integers"):
+        msm_pippenger_gpu([G.x, 1.5], [G.y], [1, 2, 3])
+    with pytest.raises(TypeError, match="points_y must contain integers"):
+        msm_pippenger_gpu([G.x], [G.y, "a"], [1, 2, 3])
+    with pytest.raises(TypeError, match="scalars must contain integers"):
+        msm_pippenger_gpu([G.x], [G.y, G.y], [1.0])
+    with pytest.raises(ValueError, match="Length mismatch: 1 x-coordinates, 2 y-coordinates, 3 scalars"):
+        msm_pippenger_gpu([G.x], [G.y, G.y], [1, 2, 3])
+    with pytest.raises(ValueError, match="Length mismatch: 2 x-coordinates, 2 y-coordinates, 1 scalars"):
+        msm_pippenger_gpu([G.x, G.x], [G.y, G.y], np.zeros((1, 4), dtype=np.uint64))
+
+
+def test_public_int_path_input_kinds():
+    # generators, object arrays, unreduced and negative coordinates, scalars of every range, and
+    # scalars as plain uint64 limbs: always (sum s_i k_i mod r) G
+    keys = [3, 5, 7, 11, 13]
+    pts = [G * k for k in keys]
+    scalars = [2**300 + 1, -9, R + 4, True, 2**256 - 1]
+    expected = (G * (sum(s * k for s, k in zip(scalars, keys)) % R)).affine()
+    xs = [p.x + (P if i % 2 else 0) for i, p in enumerate(pts)]  # below 2^256, reduced on the device
+    ys = [p.y - (P if i % 3 == 0 else 0) for i, p in enumerate(pts)]  # negative: reduced on the host
+    got = msm_pippenger_gpu((x for x in xs), np.array(ys, dtype=object), iter(scalars))
+    assert got.affine() == expected
+    limbs = ints_to_limbs([s % R for s in scalars])
+    assert msm_pippenger_gpu(xs, ys, limbs).affine() == expected
+    assert msm_pippenger_gpu(xs, ys, gb.asarray(limbs)).affine() == expected
+

@@ -163,6 +163,19 @@ int zk_g1_on_curve(const uint64_t* points_affine, int64_t n, int64_t* first_bad,
   return 0;
 }
 
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+int zk_g1_on_curve_async(const uint64_t* points_affine, int64_t n, uint64_t* first_bad_dev, void* stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(first_bad_dev);
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, stream, d, (unsigned long long)(n > 0 ? n : 0));
+  ZK_LAUNCH_CHECK("on-curve check");
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_on_curve, dim3(grid_for(n, 256)), dim3(256), 0, stream, points_affine, n, d);
+  ZK_LAUNCH_CHECK("on-curve check");
+  return 0;
+}
+
 int zk_g1_fixed_base(const uint64_t* k, int64_t n, uint64_t* out_affine, void* stream) {
   if (n <= 0) return 0;
   const Aff* table = nullptr;

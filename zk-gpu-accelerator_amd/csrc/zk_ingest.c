@@ -16,19 +16,50 @@
  *   workers only read exact ints in [0, 2^256) (type, sign, bit length and digits, none of which
  *   allocates or raises) and leave every other element to the calling thread, which converts
  *   those afterwards in index order with the interpreter's own functions.
+ * rows_into(values, modulus, name, dest, stride, offset[, start, stop]) -> (count, items)
+ *   The same conversion written into a writable buffer (page-locked staging memory for an upload):
+ *   row i goes to 32-byte row offset + i * stride of dest. With start and stop, only the elements
+ *   [start, stop) of the sequence (clamped to its length) are converted, so that a caller can
+ *   upload one part while the next is converted; count is always the sequence's length.
+ *   ValueError if dest is too small.
  * ints_from_rows(buffer) -> list of the 256-bit little-endian values of 32-byte rows.
  */
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
 #include <pthread.h>
+#include <stdint.h>
 #include <stdlib.h>
+#include <string.h>
 #include <unistd.h>
+
+/* CPython before 3.12 keeps an int as Py_SIZE(v) 30-bit digits (the sign in the size): reading them
+ * directly packs a row in a few shifts instead of a general byte-array conversion. */
+#if PY_VERSION_HEX < 0x030C0000 && PYLONG_BITS_IN_DIGIT == 30 && PY_LITTLE_ENDIAN
+#include <longintrepr.h>
+#define ZK_INGEST_DIGITS 1
+#endif
 
 /* 1 if v is an exact int in [0, 2^256), written to out; 0 if it needs the interpreter. */
 static int fast_row(PyObject* v, unsigned char* out) {
-  if (!PyLong_CheckExact(v) || _PyLong_Sign(v) < 0) return 0;
+  if (!PyLong_CheckExact(v)) return 0;
+#ifdef ZK_INGEST_DIGITS
+  const Py_ssize_t size = Py_SIZE(v);
+  if (size < 0 || size > 9) return 0; /* negative, or at least 2^270 */
+  const digit* d = ((PyLongObject*)v)->ob_digit;
+  if (size == 9 && (d[8] >> 16)) return 0; /* at least 2^256 */
+  uint64_t w[5] = {0, 0, 0, 0, 0};
+  for (Py_ssize_t i = 0; i < size; ++i) {
+    const unsigned bit = 30u * (unsigned)i, q = bit >> 6, r = bit & 63u;
+    w[q] |= (uint64_t)d[i] << r;
+    if (r > 34) w[q + 1] |= (uint64_t)d[i] >> (64u - r);
+  }
+  memcpy(out, w, 32);
+  return 1;
+#else
+  if (_PyLong_Sign(v) < 0) return 0;
   if (_PyLong_NumBits(v) > 256) return 0;
   return _PyLong_AsByteArray((PyLongObject*)v, out, 32, 1, 0) == 0;
+#endif
 }
 
 static int slow_row(PyObject* item, PyObject* modulus, const char* name, unsigned char* out) {
@@ -63,14 +94,16 @@ static int slow_row(PyObject* item, PyObject* modulus, const char* name, unsigne
 
 typedef struct {
   PyObject** items;
-  unsigned char* buf;
+  unsigned char* buf;  /* row i at buf + 32 * stride * i */
+  Py_ssize_t stride;
   unsigned char* done; /* per element: 1 when a worker wrote its row */
   Py_ssize_t lo, hi;
 } Span;
 
 static void* convert_span(void* arg) {
   Span* s = (Span*)arg;
-  for (Py_ssize_t i = s->lo; i < s->hi; ++i) s->done[i] = (unsigned char)fast_row(s->items[i], s->buf + 32 * i);
+  const Py_ssize_t step = 32 * s->stride;
+  for (Py_ssize_t i = s->lo; i < s->hi; ++i) s->done[i] = (unsigned char)fast_row(s->items[i], s->buf + step * i);
   return NULL;
 }
 
@@ -86,34 +119,20 @@ static int ingest_threads(Py_ssize_t n) {
   return t < 1 ? 1 : (int)t;
 }
 
-static PyObject* rows_from_ints(PyObject* self, PyObject* args) {
-  (void)self;
-  PyObject *values, *modulus;
-  const char* name;
-  if (!PyArg_ParseTuple(args, "OOs", &values, &modulus, &name)) return NULL;
-  if (!PyLong_Check(modulus) || _PyLong_Sign(modulus) <= 0) {
-    PyErr_SetString(PyExc_ValueError, "modulus must be a positive int");
-    return NULL;
-  }
-  PyObject* items = PySequence_Fast(values, "values must be iterable");
-  if (!items) return NULL;
-  const Py_ssize_t n = PySequence_Fast_GET_SIZE(items);
-  PyObject* out = PyBytes_FromStringAndSize(NULL, 32 * n);
+/* Converts the n items into rows buf + 32 * stride * i; 0, or -1 with the exception set. */
+static int convert_items(PyObject** it, Py_ssize_t n, PyObject* modulus, const char* name, unsigned char* buf,
+                         Py_ssize_t stride) {
   unsigned char* done = n ? (unsigned char*)calloc((size_t)n, 1) : NULL;
-  if (!out || (n && !done)) {
-    Py_XDECREF(out);
-    Py_DECREF(items);
-    free(done);
-    return PyErr_NoMemory();
+  if (n && !done) {
+    PyErr_NoMemory();
+    return -1;
   }
-  unsigned char* buf = (unsigned char*)PyBytes_AS_STRING(out);
-  PyObject** it = PySequence_Fast_ITEMS(items);
   const int nt = ingest_threads(n);
   Span spans[kMaxThreads];
   pthread_t tids[kMaxThreads];
   int started = 0;
   for (int t = 0; t < nt; ++t) {
-    spans[t] = (Span){it, buf, done, n * t / nt, n * (t + 1) / nt};
+    spans[t] = (Span){it, buf, stride, done, n * t / nt, n * (t + 1) / nt};
     if (t > 0 && pthread_create(&tids[t], NULL, convert_span, &spans[t]) == 0) ++started;
     else if (t > 0) break; /* fewer workers: the calling thread's second pass converts the rest */
   }
@@ -121,16 +140,84 @@ static PyObject* rows_from_ints(PyObject* self, PyObject* args) {
   for (int t = 1; t <= started; ++t) pthread_join(tids[t], NULL);
   for (Py_ssize_t i = 0; i < n; ++i) {
     if (done[i]) continue;
-    if (slow_row(it[i], modulus, name, buf + 32 * i) < 0) {
+    if (slow_row(it[i], modulus, name, buf + 32 * stride * i) < 0) {
       free(done);
-      Py_DECREF(out);
-      Py_DECREF(items);
-      return NULL;
+      return -1;
     }
   }
   free(done);
+  return 0;
+}
+
+static int check_modulus(PyObject* modulus) {
+  if (!PyLong_Check(modulus) || _PyLong_Sign(modulus) <= 0) {
+    PyErr_SetString(PyExc_ValueError, "modulus must be a positive int");
+    return -1;
+  }
+  return 0;
+}
+
+static PyObject* rows_from_ints(PyObject* self, PyObject* args) {
+  (void)self;
+  PyObject *values, *modulus;
+  const char* name;
+  if (!PyArg_ParseTuple(args, "OOs", &values, &modulus, &name)) return NULL;
+  if (check_modulus(modulus) < 0) return NULL;
+  PyObject* items = PySequence_Fast(values, "values must be iterable");
+  if (!items) return NULL;
+  const Py_ssize_t n = PySequence_Fast_GET_SIZE(items);
+  PyObject* out = PyBytes_FromStringAndSize(NULL, 32 * n);
+  if (!out) {
+    Py_DECREF(items);
+    return NULL;
+  }
+  if (convert_items(PySequence_Fast_ITEMS(items), n, modulus, name, (unsigned char*)PyBytes_AS_STRING(out), 1) < 0) {
+    Py_DECREF(out);
+    Py_DECREF(items);
+    return NULL;
+  }
   PyObject* res = PyTuple_Pack(2, out, items);
   Py_DECREF(out);
+  Py_DECREF(items);
+  return res;
+}
+
+static PyObject* rows_into(PyObject* self, PyObject* args) {
+  (void)self;
+  PyObject *values, *modulus, *dest;
+  const char* name;
+  Py_ssize_t stride, offset, start = 0, stop = PY_SSIZE_T_MAX;
+  if (!PyArg_ParseTuple(args, "OOsOnn|nn", &values, &modulus, &name, &dest, &stride, &offset, &start, &stop)) return NULL;
+  if (check_modulus(modulus) < 0) return NULL;
+  if (stride < 1 || offset < 0 || start < 0) {
+    PyErr_SetString(PyExc_ValueError, "stride must be positive, offset and start non-negative");
+    return NULL;
+  }
+  PyObject* items = PySequence_Fast(values, "values must be iterable");
+  if (!items) return NULL;
+  const Py_ssize_t n = PySequence_Fast_GET_SIZE(items);
+  if (stop > n) stop = n;
+  if (start > stop) start = stop;
+  Py_buffer view;
+  if (PyObject_GetBuffer(dest, &view, PyBUF_WRITABLE | PyBUF_C_CONTIGUOUS) < 0) {
+    Py_DECREF(items);
+    return NULL;
+  }
+  const Py_ssize_t rows = view.len / 32;
+  if (n && offset + (n - 1) * stride >= rows) {
+    PyBuffer_Release(&view);
+    Py_DECREF(items);
+    PyErr_SetString(PyExc_ValueError, "rows_into: the destination is too small");
+    return NULL;
+  }
+  const int rc = convert_items(PySequence_Fast_ITEMS(items) + start, stop - start, modulus, name,
+                               (unsigned char*)view.buf + 32 * (offset + start * stride), stride);
+  PyBuffer_Release(&view);
+  if (rc < 0) {
+    Py_DECREF(items);
+    return NULL;
+  }
+  PyObject* res = Py_BuildValue("(nO)", n, items);
   Py_DECREF(items);
   return res;
 }
@@ -166,6 +253,7 @@ static PyObject* ints_from_rows(PyObject* self, PyObject* arg) {
 
 static PyMethodDef kMethods[] = {
     {"rows_from_ints", rows_from_ints, METH_VARARGS, "(values, modulus, name) -> (32-byte rows as bytes, items)"},
+    {"rows_into", rows_into, METH_VARARGS, "(values, modulus, name, dest, stride, offset) -> (count, items)"},
     {"ints_from_rows", ints_from_rows, METH_O, "buffer of 32-byte rows -> list of ints"},
     {NULL, NULL, 0, NULL},
 };

@@ -257,6 +257,67 @@ int zk_memcpy_h2d(void* dst, const void* src, uint64_t bytes, void* stream) {
   return 0;
 }
 
+// Pinned host blocks: freed blocks stay pinned in a cache of at most kHostCacheBytes (pinning
+// costs milliseconds per hundred MB, so the public calls reuse their staging blocks).
+namespace {
+constexpr size_t kHostCacheBytes = (size_t)2 << 30;
+std::mutex g_host_mu;
+std::map<void*, size_t> g_host_live;        // block -> size
+std::multimap<size_t, void*> g_host_cache;  // size -> block
+size_t g_host_cached = 0;
+}  // namespace
+
+int zk_host_alloc(void** ptr, uint64_t bytes) {
+  *ptr = nullptr;
+  const size_t want = std::max<size_t>((size_t)bytes, 1);
+  {
+    std::lock_guard<std::mutex> lock(g_host_mu);
+    auto it = g_host_cache.lower_bound(want);
+    if (it != g_host_cache.end() && it->first <= 2 * want) {  // reuse a block not much larger
+      *ptr = it->second;
+      g_host_live[it->second] = it->first;
+      g_host_cached -= it->first;
+      g_host_cache.erase(it);
+      return 0;
+    }
+  }
+  const size_t size = (want + 0xfffff) & ~(size_t)0xfffff;  // whole MiB
+  void* p = nullptr;
+  ZK_HIP(hipHostMalloc(&p, size, hipHostMallocDefault));
+  std::lock_guard<std::mutex> lock(g_host_mu);
+  g_host_live[p] = size;
+  *ptr = p;
+  return 0;
+}
+
+int zk_host_free(void* ptr) {
+  if (!ptr) return 0;
+  std::vector<void*> drop;
+  {
+    std::lock_guard<std::mutex> lock(g_host_mu);
+    auto it = g_host_live.find(ptr);
+    if (it == g_host_live.end()) return fail("zk_host_free: not a block from zk_host_alloc");
+    const size_t size = it->second;
+    g_host_live.erase(it);
+    g_host_cache.emplace(size, ptr);
+    g_host_cached += size;
+    while (g_host_cached > kHostCacheBytes) {  // the smallest blocks go first
+      auto small = g_host_cache.begin();
+      g_host_cached -= small->first;
+      drop.push_back(small->second);
+      g_host_cache.erase(small);
+    }
+  }
+  for (void* p : drop) ZK_HIP(hipHostFree(p));
+  return 0;
+}
+
+int zk_memcpy_h2d_async(void* dst, const void* src, uint64_t bytes, void* stream) {
+  if (!bytes) return 0;
+  ZK_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  return 0;
+}
+
 int zk_memcpy_d2h(void* dst, const void* src, uint64_t bytes, void* stream) {
   if (!bytes) return 0;
   ZK_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));

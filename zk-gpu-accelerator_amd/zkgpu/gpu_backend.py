@@ -67,6 +67,9 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_memcpy_d2h": [_vp, _vp, _u64, _vp],
     "zk_memcpy_d2d": [_vp, _vp, _u64, _vp],
     "zk_memset": [_vp, _i32, _u64, _vp],
+    "zk_host_alloc": [ctypes.POINTER(_vp), _u64],
+    "zk_host_free": [_vp],
+    "zk_memcpy_h2d_async": [_vp, _vp, _u64, _vp],
     "zk_copy": [_vp, _vp, _u64, _vp],
     "zk_stream_create": [ctypes.POINTER(_vp)],
     "zk_stream_destroy": [_vp],
@@ -118,6 +121,7 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_g1_add_affine": [_vp, _i32, _vp, _i32, _vp, _i64, _vp],
     "zk_g1_double": [_vp, _vp, _i64, _vp],
     "zk_g1_on_curve": [_vp, _i64, _i64p, _vp],
+    "zk_g1_on_curve_async": [_vp, _i64, _vp, _vp],
     "zk_g1_fixed_base": [_vp, _i64, _vp, _vp],
     "zk_msm_g1": [_vp, _vp, _i64, _i32, _i32, _vp, _vp],
     "zk_msm_g1_entries": [_vp, _vp, _i64, _i32, _i32, _vp, _u64p, _vp],
@@ -481,6 +485,48 @@ def asarray(host: Any, dtype: Any = None) -> DeviceArray:
     if out.nbytes:
         call("zk_memcpy_h2d", out.ptr, arr.ctypes.data, out.nbytes, current_stream())
     return out
+
+
+class PinnedBuffer:
+    """
+    Page-locked host staging memory (zk_host_alloc) for uploads at PCIe speed, used as a context
+    manager: `view` is a writable buffer over the block, upload() enqueues an asynchronous copy to
+    device memory on the current stream, and leaving the block waits for that stream before the
+    block goes back to the library's cache (zk_host_free).
+    """
+
+    def __init__(self, nbytes: int) -> None:
+        require_gpu("PinnedBuffer")
+        p = ctypes.c_void_p()
+        call("zk_host_alloc", ctypes.byref(p), max(int(nbytes), 1))
+        self.ptr = p.value or 0
+        self.nbytes = int(nbytes)
+        self.view = (ctypes.c_char * max(self.nbytes, 1)).from_address(self.ptr)
+
+    def upload(self, dst: DeviceArray, offset: int, nbytes: int) -> None:
+        """Copy bytes [offset, offset + nbytes) of the block to the start of dst, asynchronously."""
+        if offset < 0 or nbytes < 0 or offset + nbytes > self.nbytes or nbytes > dst.nbytes:
+            raise ValueError("PinnedBuffer.upload: range outside the block or the destination")
+        if nbytes:
+            call("zk_memcpy_h2d_async", dst.ptr, self.ptr + offset, nbytes, current_stream())
+
+    def release(self) -> None:
+        if self.ptr and LIB is not None:
+            call("zk_stream_sync", current_stream())
+            LIB.zk_host_free(self.ptr)
+        self.ptr = 0
+
+    def __enter__(self) -> "PinnedBuffer":
+        return self
+
+    def __exit__(self, *exc: Any) -> None:
+        self.release()
+
+    def __del__(self) -> None:
+        if self.ptr and LIB is not None:
+            LIB.zk_stream_sync(current_stream())
+            LIB.zk_host_free(self.ptr)
+            self.ptr = 0
 
 
 def ascontiguousarray(x: Any) -> DeviceArray:

@@ -30,7 +30,7 @@ import numpy as np
 from zkgpu import gpu_backend as _gb
 from zkgpu.curve import P as _P, Point, from_jacobian
 from zkgpu.field import FIELD_MODULUS as _GROUP_ORDER
-from zkgpu.field256_gpu import BN254_FQ, LIMBS as _FIELD_LIMBS, _limbs_to_ints, _rows_from_ints
+from zkgpu.field256_gpu import BN254_FQ, LIMBS as _FIELD_LIMBS, _ingest, _limbs_to_ints, _rows_from_ints
 from zkgpu.gpu_backend import GPU_AVAILABLE as _gpu_available, DeviceArray
 from zkgpu.msm import _check_window_size
 
@@ -172,6 +172,63 @@ def _msm_resident(points: Any, points_y: Any, scalars: Any, window: int) -> Poin
     return _result(_run(points, sc, n, window, _CHUNK_LEN))
 
 
+def _as_sequence(values: Any) -> Sequence[Any]:
+    return values if isinstance(values, Sequence) else list(values)
+
+
+def _msm_staged(points_x: Any, points_y: Any, scalars: Any, window: int) -> Point:
+    """msm_pippenger_gpu of host integers. C threads convert each input straight into one
+    page-locked block (x and y interleaved as (n, 2, 4) rows, then the scalar rows); the points go
+    up and are reduced, brought into Montgomery form and checked on the curve while the scalars are
+    converted, and the scalars go up in parts as they are converted. The errors and their order are
+    the reference's (msm_gpu.py:432-442): integer types of x, y and the scalars, the lengths, then
+    the first point off the curve."""
+    xs, ys = _as_sequence(points_x), _as_sequence(points_y)
+    limbs = isinstance(scalars, (DeviceArray, np.ndarray)) and getattr(scalars, "dtype", None) == np.uint64
+    ss = scalars if limbs else _as_sequence(scalars)
+    m = max(len(xs), len(ys))
+    with _gb.PinnedBuffer(64 * m + (0 if limbs else 32 * len(ss))) as buf:
+        _ingest.rows_into(xs, _P, "points_x", buf.view, 2, 0)
+        _ingest.rows_into(ys, _P, "points_y", buf.view, 2, 1)
+        if limbs and (ss.ndim != 2 or ss.shape[1] != _FIELD_LIMBS):
+            raise TypeError(f"scalars must be an (n, 4) uint64 array of plain limbs, got shape {ss.shape}")
+        n = int(ss.shape[0]) if limbs else len(ss)
+        points = bad = sc = None
+        if len(xs) == len(ys) == n > 0:
+            # the points' device work runs while the scalars are converted
+            _kernels()
+            raw = _gb.empty((2 * n, _FIELD_LIMBS))
+            buf.upload(raw, 0, 64 * n)
+            points = BN254_FQ.from_limbs(raw).reshape(n, 2, _FIELD_LIMBS)
+            bad = _gb.empty((1,))
+            _gb.call("zk_g1_on_curve_async", points.ptr, n, bad.ptr, _gb.current_stream())
+            sc = _gb.empty((n, _FIELD_LIMBS))
+        if limbs:
+            pass
+        elif sc is None:
+            _ingest.rows_into(ss, _GROUP_ORDER, "scalars", buf.view, 1, 2 * m)
+        else:
+            part = max(1 << 16, -(-n // 4))
+            for lo in range(0, n, part):
+                hi = min(n, lo + part)
+                _ingest.rows_into(ss, _GROUP_ORDER, "scalars", buf.view, 1, 2 * m, lo, hi)
+                buf.upload(sc[lo:hi], 64 * m + 32 * lo, 32 * (hi - lo))
+        if not len(xs) == len(ys) == n:
+            raise ValueError(f"Length mismatch: {len(xs)} x-coordinates, {len(ys)} y-coordinates, {n} scalars.")
+        if n == 0:
+            return Point.identity()
+        if limbs:
+            sc = _gb.asarray(np.ascontiguousarray(ss) if isinstance(ss, np.ndarray) else ss)
+        out = _gb.empty((3, _FIELD_LIMBS))
+        w, chunk = _tuned(n, window, _CHUNK_LEN)
+        _gb.call("zk_msm_g1", points.ptr, sc.ptr, n, w, chunk, out.ptr, _gb.current_stream())
+        first_bad = int(bad.get()[0])
+        if first_bad < n:
+            x, y = operator.index(xs[first_bad]) % _P, operator.index(ys[first_bad]) % _P
+            raise ValueError(f"({x}, {y}) is not on the BN254 curve y^2 = x^3 + 3")
+        return _result(out)
+
+
 def msm_pippenger_gpu(
     points_x: Iterable[int] | Any,
     points_y: Iterable[int] | None,
@@ -205,6 +262,8 @@ def msm_pippenger_gpu(
 
     if isinstance(points_x, (SRS, DeviceArray)):
         return _msm_resident(points_x, points_y, scalars, window_size or 0)
+    if _gpu_available and _ingest is not None:
+        return _msm_staged(points_x, points_y, scalars, window_size or 0)
     xrows, xs = _rows_from_ints(points_x, _P, "points_x")
     yrows, ys = _rows_from_ints(points_y, _P, "points_y")
     if isinstance(scalars, (DeviceArray, np.ndarray)) and getattr(scalars, "dtype", None) == np.uint64:
