@@ -79,6 +79,9 @@ def main() -> int:
                     help="an extra Poisson arrival rate (jobs/s) beside 0.5, 0.8 and 0.95 of the measured capacity")
     ap.add_argument("--serve-budget-gib", type=float, default=128.0, help="HBM budget of the server")
     ap.add_argument("--plain-sample", type=int, default=200, help="jobs timed through the per-call path")
+    ap.add_argument("--msm-sizes", metavar="LOGS",
+                    help="only plain MSMs at these log2 sizes (comma-separated), --warmup untimed and --steps timed "
+                         "each (for a per-size kernel trace); one JSON line")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="copy 256 MiB with zk_copy first (the known-size kernel for scaling byte counters)")
     ap.add_argument("--suite", metavar="JSON",
@@ -102,6 +105,10 @@ def main() -> int:
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     if args.suite:
         _reference_suite(Path(args.suite), gb, log)
+        return 0
+
+    if args.msm_sizes:
+        print(json.dumps(_msm_sizes(args, gb, workloads, msm_g1_jacobian, log)), flush=True)
         return 0
 
     if args.ntt64_only:
@@ -250,6 +257,21 @@ def _public_call(gb, points, scalars_host, resident_ms: float, log) -> dict:
     return {"msm_public_python_ints": {"n": n, "ms": round(ms, 2), "vs_resident": round(ms / resident_ms, 1),
                                        "note": "msm_pippenger_gpu(xs, ys, scalars) with Python ints: host conversion, "
                                                "PCIe upload, on-device validation, MSM, result as a Point"}}
+
+
+def _msm_sizes(args, gb, workloads, msm_g1_jacobian, log) -> dict:
+    """Plain MSMs at several sizes (P_i = k_i G, k_i seeded 0x5EED02, witness-like scalars), each
+    timed alone: the latency of one MSM by size."""
+    res = {}
+    for lg in (int(v) for v in args.msm_sizes.split(",")):
+        n = 1 << lg
+        pts = workloads.srs_points(workloads.key_rows(n, 0x5EED02))
+        sc = gb.asarray(workloads.witness_rows_fast(n, 0x5EED02))
+        out = gb.empty((3, 4))
+        ms = timed(lambda: msm_g1_jacobian(pts, sc, out), args.steps, args.warmup, gb)
+        log(f"msm 2^{lg}: {ms:.4f} ms")
+        res[str(lg)] = round(ms, 4)
+    return {"msm_ms_by_log_size": res}
 
 
 def _msm_2p16(args, gb, workloads, msm_g1_jacobian, log) -> dict:
