@@ -48,7 +48,10 @@
 //              groups of windows side by side, all with the limb-sliced formulas of
 //              zk_curve_sliced.hpp, and the Jacobian result stays on the device (the
 //              reference combines windows on the CPU, msm_gpu.py:449 and
-//              msm.py:156-169).
+//              msm.py:156-169). Terms, window totals and the final combination are one
+//              launch: the last workgroup to finish a window (or a job) goes on.
+//
+// Steps 1 and 2 and the clearing of the buckets share the plan's first launch (k_prepare).
 //
 // The result is a unique group element, so it equals the reference's
 // msm_pippenger for every input; intermediate coordinates never leave the device.
@@ -277,11 +280,9 @@ __device__ __forceinline__ Jac load_bucket(const F& f, const BucketRec* p) {
 
 // ---- kernels ----------------------------------------------------------------
 
-// Points (256-form rows) -> 261-form records; with phi, also phi(P_i) = (beta x_i, y_i) at phi + i.
-__global__ void __launch_bounds__(256) k_prep_points(const uint64_t* __restrict__ pts, int64_t n, AffRec* __restrict__ out,
-                                                     AffRec* __restrict__ phi) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// Point i (a 256-form row pair) -> its 261-form record; with phi, also phi(P_i) = (beta x_i, y_i).
+__device__ __forceinline__ void prep_point(const uint64_t* __restrict__ pts, int64_t i, AffRec* __restrict__ out,
+                                           AffRec* __restrict__ phi) {
   const Fq f;
   Aff a = load_aff(f, pts + 8 * i);
   store_aff_rec(out + i, a);
@@ -289,6 +290,12 @@ __global__ void __launch_bounds__(256) k_prep_points(const uint64_t* __restrict_
     a.x = mont(f, a.x, fe_const<Fq>(Glv::beta261));
     store_aff_rec(phi + i, a);
   }
+}
+
+__global__ void __launch_bounds__(256) k_prep_points(const uint64_t* __restrict__ pts, int64_t n, AffRec* __restrict__ out,
+                                                     AffRec* __restrict__ phi) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) prep_point(pts, i, out, phi);
 }
 
 // Per-MSM parameters of a batch (see MsmPlan).
@@ -330,11 +337,11 @@ __device__ __forceinline__ int find_job(const MsmJob* jobs, int njobs, int64_t x
 
 // Signed digits of every scalar: digits[dig_base + w * nv + v] = d (0 for none), v = i for the
 // scalar itself, or with the GLV split v = i for k1 (point P_i) and v = n + i for k2 (phi(P_i)),
-// the sign of k1 / k2 folded into the digits.
-__global__ void __launch_bounds__(256) k_signed_digits(const MsmJob* __restrict__ jobs, int njobs,
-                                                       int32_t* __restrict__ digits) {
-  const MsmJob& job = jobs[find_job<&MsmJob::block_base>(jobs, njobs, blockIdx.x)];
-  const int64_t i = ((int64_t)blockIdx.x - job.block_base) * 256 + threadIdx.x;
+// the sign of k1 / k2 folded into the digits. Block b of the digit grid (256 scalars).
+__device__ __forceinline__ void signed_digits_block(const MsmJob* __restrict__ jobs, int njobs, int64_t b,
+                                                    int32_t* __restrict__ digits) {
+  const MsmJob& job = jobs[find_job<&MsmJob::block_base>(jobs, njobs, b)];
+  const int64_t i = (b - job.block_base) * 256 + threadIdx.x;
   if (i >= job.n) return;
   uint32_t w[8];
   {
@@ -358,6 +365,19 @@ __global__ void __launch_bounds__(256) k_signed_digits(const MsmJob* __restrict_
     d[(int64_t)win * job.nv + i] = neg1 ? -d1 : d1;
     d[(int64_t)win * job.nv + job.n + i] = neg2 ? -d2 : d2;
   }
+}
+
+// The first launch of a plan, three independent jobs in one grid: the signed digits (blocks below
+// `digit_blocks`), the bucket records cleared to zero (one 32-bit word per thread), and, for a plan
+// over raw points, the point records (one point per thread). Grid-stride, so any grid covers them.
+__global__ void __launch_bounds__(256) k_prepare(const MsmJob* __restrict__ jobs, int njobs, int64_t digit_blocks,
+                                                 int32_t* __restrict__ digits, uint32_t* __restrict__ clear,
+                                                 int64_t clear_words, const uint64_t* __restrict__ pts, int64_t npts,
+                                                 AffRec* __restrict__ out, AffRec* __restrict__ phi) {
+  const int64_t threads = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < npts; i += threads) prep_point(pts, i, out, phi);
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < clear_words; i += threads) clear[i] = 0;
+  for (int64_t b = blockIdx.x; b < digit_blocks; b += gridDim.x) signed_digits_block(jobs, njobs, b, digits);
 }
 
 // The sort runs in two passes so that its writes stay local. Pass A is a counting sort of the
@@ -1105,14 +1125,14 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
   if (threadIdx.x == 0) parts[blockIdx.x] = shj[0];
 }
 
-// Window sums U = sum T_s + sum_j 2^(j + log_seg) Q_j, in two launches. k_window_terms: four
-// workgroups of four wavefronts per (job, window), one wavefront per SIMD (the limb-sliced formulas
-// keep a SIMD's issue busy with one wavefront, so a second one there would only wait). Wavefront
-// slot i of the window takes the terms i, i + 16, ...: a term is the sum of its reduction row's
-// chunk partials; term j < log_segs is Q_j doubled j + log_seg times, the last two are the halves
-// of sum T_s. The doublings of different terms run side by side (a Horner pass over j would chain
-// all of them with the additions); each workgroup sums its wavefronts' terms, and k_window_total
-// sums a window's four partials.
+// The window sums U = sum T_s + sum_j 2^(j + log_seg) Q_j and the final combination, in one launch
+// (k_window_tail). Terms: four workgroups of four wavefronts per (job, window), one wavefront per
+// SIMD (the limb-sliced formulas keep a SIMD's issue busy with one wavefront, so a second one
+// there would only wait). Wavefront slot i of the window takes the terms i, i + 16, ...: a term is
+// the sum of its reduction row's chunk partials; term j < log_segs is Q_j doubled j + log_seg
+// times, the last two are the halves of sum T_s. The doublings of different terms run side by
+// side (a Horner pass over j would chain all of them with the additions); each workgroup sums its
+// wavefronts' terms, and the window's last workgroup adds the four partials.
 constexpr int kWinGroups = 4;
 constexpr int kWinWaves = 4;
 
@@ -1134,72 +1154,84 @@ __device__ __forceinline__ SlJac sl_workgroup_sum(JacRec* sh, const SlLane& c, c
   return sl_from_jac(load_jac_rec(&sh[0]));
 }
 
-__global__ void __launch_bounds__(64 * kWinWaves) k_window_terms(const MsmJob* __restrict__ jobs, int njobs,
-                                                                 int64_t total_windows, const JacRec* __restrict__ parts,
-                                                                 JacRec* __restrict__ wparts) {
+// Final combination, by the workgroup of a job's last window: sum_w 2^(cw) U_w with the limb-sliced
+// formulas. The windows split into kFinalWaves contiguous groups, the top one the smallest;
+// wavefront g runs Horner over its group [lo, hi) from the top down, then doubles its sum c * lo
+// times, and the groups' results meet in a two-level tree. Every path keeps the (rw - 1) c
+// doublings a single Horner chain has (the top window's), but only the top group's additions and
+// the tree's stay on it: at 2^10 points (22 windows of 6 bits) 5 additions against 21.
+constexpr int kFinalWaves = 4;
+
+// Stages of k_window_tail follow each other without a launch between them: every workgroup counts
+// itself in its window's counter once its partial is stored; the last of a window's kWinGroups
+// workgroups adds the window's partials and counts the window in its job's counter; the last
+// window of a job runs the job's final combination. The counters start at zero (k_prepare clears
+// them with the buckets), and device-scope fences around the atomic increments make each stage's
+// stores visible to the workgroup that reads them, on whatever XCD it runs. Nothing waits: a
+// workgroup that is not last simply ends.
+__device__ __forceinline__ bool last_arrival(uint32_t* counter, uint32_t expected, uint32_t* ticket) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    *ticket = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const bool last = *ticket == expected - 1;
+  if (last) __threadfence();
+  return last;
+}
+
+__global__ void __launch_bounds__(64 * kWinWaves) k_window_tail(const MsmJob* __restrict__ jobs, int njobs,
+                                                                int64_t total_windows, const JacRec* __restrict__ parts,
+                                                                JacRec* __restrict__ wparts, JacRec* __restrict__ wsum,
+                                                                uint32_t* __restrict__ counters) {
+  static_assert(kWinGroups == kWinWaves && kFinalWaves == kWinWaves, "one workgroup shape for the three stages");
   __shared__ JacRec sh[kWinWaves];
+  __shared__ uint32_t ticket;
   const int64_t g = blockIdx.x / kWinGroups;
   const int group = (int)(blockIdx.x % kWinGroups);
   if (g >= total_windows) return;
-  const MsmJob& job = jobs[find_job<&MsmJob::win_base>(jobs, njobs, g)];
-  const int64_t win = g - job.win_base;
-  const int per = red_rows(job);
-  const JacRec* p = parts + job.part_base + win * per * job.chunks;
-  const FqConst f;
-  const SlLane c = sl_lane<FqConst>();
-  SlJac acc = sl_from_jac(jac_identity(f));
-  for (int t = group * kWinWaves + ((int)threadIdx.x >> 6); t < per; t += kWinGroups * kWinWaves) {
-    SlJac d = sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks));
-    for (int k = 1; k < job.chunks; ++k) d = sl_add<FqConst>(c, d, sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks + k)));
-    const int dbl = t < job.log_segs ? t + job.log_seg : 0;
-    for (int e = 0; e < dbl; ++e) d = sl_dbl<FqConst>(c, d);
-    acc = sl_add<FqConst>(c, acc, d);
-  }
-  acc = sl_workgroup_sum<kWinWaves>(sh, c, acc);
-  const Jac r = sl_to_jac<FqConst>(c, acc);
-  if (threadIdx.x == 0) store_jac_rec(wparts + blockIdx.x, r);
-}
-
-// One workgroup per (job, window): the sum of the window's kWinGroups partials.
-__global__ void __launch_bounds__(64 * kWinGroups) k_window_total(int64_t total_windows, const JacRec* __restrict__ wparts,
-                                                                  JacRec* __restrict__ wsum) {
-  __shared__ JacRec sh[kWinGroups];
-  const int64_t g = blockIdx.x;
-  if (g >= total_windows) return;
-  const SlLane c = sl_lane<FqConst>();
-  const SlJac mine = sl_from_jac(load_jac_rec(wparts + g * kWinGroups + ((int)threadIdx.x >> 6)));
-  const Jac r = sl_to_jac<FqConst>(c, sl_workgroup_sum<kWinGroups>(sh, c, mine));
-  if (threadIdx.x == 0) store_jac_rec(wsum + g, r);
-}
-
-// One workgroup per job: sum_w 2^(cw) U_w with the limb-sliced formulas. The windows split into
-// kFinalWaves contiguous groups, the top one the smallest; wavefront g runs Horner over its group
-// [lo, hi) from the top down, then doubles its sum c * lo times, and the groups' results meet in a
-// two-level tree. Every path keeps the (rw - 1) c doublings a single Horner chain has (the top
-// window's), but only the top group's additions and the tree's stay on it: at 2^10 points (22
-// windows of 6 bits) 5 additions against 21.
-constexpr int kFinalWaves = 4;
-
-__global__ void __launch_bounds__(64 * kFinalWaves) k_final(const MsmJob* __restrict__ jobs, int njobs,
-                                                            const JacRec* __restrict__ wsum) {
-  __shared__ JacRec sh[kFinalWaves];
-  const int jb = blockIdx.x;
-  if (jb >= njobs) return;
+  const int jb = find_job<&MsmJob::win_base>(jobs, njobs, g);
   const MsmJob& job = jobs[jb];
   const FqConst f;
   const SlLane c = sl_lane<FqConst>();
-  const int per = (job.rw + kFinalWaves - 1) / kFinalWaves;
-  const int lo = min(job.rw, ((int)threadIdx.x >> 6) * per), hi = min(job.rw, lo + per);
-  SlJac acc = sl_from_jac(hi > lo ? load_jac_rec(wsum + job.win_base + hi - 1) : jac_identity(f));
-  for (int w = hi - 2; w >= lo; --w) {
-    for (int e = 0; e < job.c; ++e) acc = sl_dbl<FqConst>(c, acc);
-    acc = sl_add<FqConst>(c, acc, sl_from_jac(load_jac_rec(wsum + job.win_base + w)));
+  {  // this workgroup's terms of window g
+    const int64_t win = g - job.win_base;
+    const int per = red_rows(job);
+    const JacRec* p = parts + job.part_base + win * per * job.chunks;
+    SlJac acc = sl_from_jac(jac_identity(f));
+    for (int t = group * kWinWaves + ((int)threadIdx.x >> 6); t < per; t += kWinGroups * kWinWaves) {
+      SlJac d = sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks));
+      for (int k = 1; k < job.chunks; ++k) d = sl_add<FqConst>(c, d, sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks + k)));
+      const int dbl = t < job.log_segs ? t + job.log_seg : 0;
+      for (int e = 0; e < dbl; ++e) d = sl_dbl<FqConst>(c, d);
+      acc = sl_add<FqConst>(c, acc, d);
+    }
+    acc = sl_workgroup_sum<kWinWaves>(sh, c, acc);
+    const Jac r = sl_to_jac<FqConst>(c, acc);
+    if (threadIdx.x == 0) store_jac_rec(wparts + blockIdx.x, r);
   }
-  if (hi > lo)
-    for (int e = job.c * lo; e > 0; --e) acc = sl_dbl<FqConst>(c, acc);
-  Jac r = sl_to_jac<FqConst>(c, sl_workgroup_sum<kFinalWaves>(sh, c, acc));
-  if (fe_is_zero_lazy(f, r.z)) r = jac_identity(f);
-  if (threadIdx.x == 0) store_jac(f, job.out, r);
+  if (!last_arrival(counters + g, kWinGroups, &ticket)) return;
+  {  // the window's total
+    const SlJac mine = sl_from_jac(load_jac_rec(wparts + g * kWinGroups + ((int)threadIdx.x >> 6)));
+    const Jac r = sl_to_jac<FqConst>(c, sl_workgroup_sum<kWinGroups>(sh, c, mine));
+    if (threadIdx.x == 0) store_jac_rec(wsum + g, r);
+  }
+  if (!last_arrival(counters + total_windows + jb, (uint32_t)job.rw, &ticket)) return;
+  {  // the job's final combination
+    const int per = (job.rw + kFinalWaves - 1) / kFinalWaves;
+    const int lo = min(job.rw, ((int)threadIdx.x >> 6) * per), hi = min(job.rw, lo + per);
+    SlJac acc = sl_from_jac(hi > lo ? load_jac_rec(wsum + job.win_base + hi - 1) : jac_identity(f));
+    for (int w = hi - 2; w >= lo; --w) {
+      for (int e = 0; e < job.c; ++e) acc = sl_dbl<FqConst>(c, acc);
+      acc = sl_add<FqConst>(c, acc, sl_from_jac(load_jac_rec(wsum + job.win_base + w)));
+    }
+    if (hi > lo)
+      for (int e = job.c * lo; e > 0; --e) acc = sl_dbl<FqConst>(c, acc);
+    Jac r = sl_to_jac<FqConst>(c, sl_workgroup_sum<kFinalWaves>(sh, c, acc));
+    if (fe_is_zero_lazy(f, r.z)) r = jac_identity(f);
+    if (threadIdx.x == 0) store_jac(f, job.out, r);
+  }
 }
 
 // Window table of an SRS: out[w * n + i] = 2^(c w) P_i as an affine record, w < windows (one
@@ -1280,6 +1312,12 @@ struct MsmPlan {
   size_t bytes = 0;
   void* srs = nullptr;  // the SRS handle the plan reads, if any
   int acc_variant = 0;  // the accumulation kernel (g_acc_variant when planned)
+  // A plan over raw (n, 2, 4) point rows converts them in its first launch: prep_n points from
+  // prep_src into prep_out[0, prep_n), and phi(P_i) after them when prep_phi.
+  const uint64_t* prep_src = nullptr;
+  int64_t prep_n = 0;
+  AffRec* prep_out = nullptr;
+  bool prep_phi = false;
   int32_t* digits = nullptr;
   const AffRec* points = nullptr;
   // device buffers
@@ -1292,6 +1330,7 @@ struct MsmPlan {
   RunSummary* runs[2] = {nullptr, nullptr};
   XyzzRec *tot = nullptr, *run = nullptr;
   JacRec *parts_buf = nullptr, *wparts = nullptr, *wsum = nullptr;
+  uint32_t* tail_counters = nullptr;  // after the buckets: one per window, then one per job
 };
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1409,7 +1448,7 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   L->chunk_base = take(4 * (size_t)(pkeys + 1));
   L->counts_b = take(4 * ((size_t)plan->max_chunks << kPartBits));
   L->counts_raw = take(4 * ((size_t)plan->max_chunks << kPartBits));
-  L->buckets = take(sizeof(BucketRec) * (size_t)keys);
+  L->buckets = take(sizeof(BucketRec) * (size_t)keys + 4 * (size_t)(wins + (int64_t)jobs.size()));  // + tail counters
   L->r0 = take(sizeof(RunSummary) * (size_t)max_slots);
   L->r1 = take(sizeof(RunSummary) * (size_t)max_slots1);
   L->tot = take(sizeof(XyzzRec) * (size_t)segs);
@@ -1440,6 +1479,7 @@ static void msm_plan_bind(MsmPlan* plan, const MsmLayout& L, void* mem) {
   plan->counts_raw = reinterpret_cast<uint32_t*>(base + L.counts_raw);
   plan->scan_ws = reinterpret_cast<uint32_t*>(base + L.scan);
   plan->buckets = reinterpret_cast<BucketRec*>(base + L.buckets);
+  plan->tail_counters = reinterpret_cast<uint32_t*>(plan->buckets + plan->nkeys);
   plan->runs[0] = reinterpret_cast<RunSummary*>(base + L.r0);
   plan->runs[1] = reinterpret_cast<RunSummary*>(base + L.r1);
   plan->tot = reinterpret_cast<XyzzRec*>(base + L.tot);
@@ -1486,10 +1526,18 @@ void msm_plan_destroy(MsmPlan* plan) {
 }
 
 // Clears the plan's buckets (every launch starts from empty ones).
-static int msm_plan_clear(MsmPlan* plan, hipStream_t stream) {
-  hipLaunchKernelGGL(k_fill_zero<uint32_t>, dim3(grid_for(plan->nkeys * 36, 256)), dim3(256), 0, stream,
-                     reinterpret_cast<uint32_t*>(plan->buckets), plan->nkeys * 36);
-  ZK_LAUNCH_CHECK("msm clear");
+// The plan's first launch: digits, the buckets cleared (a fill kernel, not a memset node, so that
+// graph replays clear them too), and the raw points converted when the plan has any.
+static int msm_plan_prepare(MsmPlan* plan, hipStream_t stream) {
+  static_assert(sizeof(BucketRec) == 36 * 4, "a bucket record is 36 words");
+  const int64_t clear_words = plan->nkeys * 36 + plan->windows_total + plan->njobs;  // buckets and tail counters
+  const int64_t grid = std::min<int64_t>(
+      std::max<int64_t>({plan->blocks, (int64_t)grid_for(clear_words, 256), (int64_t)grid_for(plan->prep_n, 256), 1}),
+      (int64_t)1 << 16);
+  hipLaunchKernelGGL(k_prepare, dim3((unsigned)grid), dim3(256), 0, stream, plan->d_jobs, plan->njobs, plan->blocks,
+                     plan->digits, reinterpret_cast<uint32_t*>(plan->buckets), clear_words, plan->prep_src,
+                     plan->prep_n, plan->prep_out, plan->prep_phi ? plan->prep_out + plan->prep_n : nullptr);
+  ZK_LAUNCH_CHECK("msm digits");
   return 0;
 }
 
@@ -1539,10 +1587,8 @@ int msm_plan_launch(MsmPlan* plan, hipStream_t stream) {
 
 static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream) {
   const int nj = plan->njobs;
-  if (int rc = msm_plan_clear(plan, stream)) return rc;
+  if (int rc = msm_plan_prepare(plan, stream)) return rc;
   {
-    hipLaunchKernelGGL(k_signed_digits, dim3((unsigned)plan->blocks), dim3(256), 0, stream, plan->d_jobs, nj, plan->digits);
-    ZK_LAUNCH_CHECK("msm digits");
     hipLaunchKernelGGL(k_sort_count, dim3((unsigned)plan->sort_blocks), dim3(256), 0, stream, plan->d_jobs, nj,
                        plan->digits, plan->counts);
     ZK_LAUNCH_CHECK("msm sort count");
@@ -1614,15 +1660,11 @@ static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream) {
     hipLaunchKernelGGL(k_reduce_bits, dim3((unsigned)plan->parts), dim3(256), 0, stream, plan->d_jobs, nj, plan->tot,
                        plan->run, plan->parts_buf);
     ZK_LAUNCH_CHECK("msm reduce bits");
-    hipLaunchKernelGGL(k_window_terms, dim3((unsigned)(plan->windows_total * kWinGroups)), dim3(64 * kWinWaves), 0, stream,
-                       plan->d_jobs, nj, plan->windows_total, plan->parts_buf, plan->wparts);
-    ZK_LAUNCH_CHECK("msm window terms");
-    hipLaunchKernelGGL(k_window_total, dim3((unsigned)plan->windows_total), dim3(64 * kWinGroups), 0, stream,
-                       plan->windows_total, plan->wparts, plan->wsum);
+    hipLaunchKernelGGL(k_window_tail, dim3((unsigned)(plan->windows_total * kWinGroups)), dim3(64 * kWinWaves), 0, stream,
+                       plan->d_jobs, nj, plan->windows_total, plan->parts_buf, plan->wparts, plan->wsum,
+                       plan->tail_counters);
     ZK_LAUNCH_CHECK("msm window sums");
   }
-  hipLaunchKernelGGL(k_final, dim3((unsigned)nj), dim3(64 * kFinalWaves), 0, stream, plan->d_jobs, nj, plan->wsum);
-  ZK_LAUNCH_CHECK("msm final");
   return 0;
 }
 
@@ -1644,7 +1686,6 @@ int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, in
   Scratch sc(stream);
   AffRec* pts = sc.get<AffRec>((size_t)std::max<int64_t>(glv ? 2 * n : n, 1));
   if (!pts) return 1;
-  if (int rc = convert_points(points, n, pts, glv, stream)) return rc;
   MsmJob job{};
   job.scalars = scalars;
   job.n = n;
@@ -1653,6 +1694,10 @@ int msm_g1(const uint64_t* points, const uint64_t* scalars, int64_t n, int c, in
   job.phi_base = n;
   MsmPlan* plan = nullptr;
   if (int rc = msm_plan_create(pts, {job}, k0, stream, &plan)) return rc;
+  plan->prep_src = points;  // the points are converted in the plan's first launch
+  plan->prep_n = n;
+  plan->prep_out = pts;
+  plan->prep_phi = glv;
   int rc = msm_plan_launch(plan, stream);
   if (rc == 0 && entries) {  // the sorted-entry count the accumulation processed
     uint32_t m = 0;
@@ -1732,8 +1777,11 @@ int msm_g1_noalloc(const uint64_t* points, const uint64_t* scalars, int64_t n, i
                 std::to_string(rec_bytes + L.total) + " bytes of scratch, more than its " +
                 std::to_string(kNoAllocScratchBytes) + " (use zk_msm_g1)");
   AffRec* pts = reinterpret_cast<AffRec*>(base);
-  if (int rc = convert_points(points, n, pts, glv, stream)) return rc;
   plan.points = pts;
+  plan.prep_src = points;  // converted in the plan's first launch
+  plan.prep_n = n;
+  plan.prep_out = pts;
+  plan.prep_phi = glv;
   msm_plan_bind(&plan, L, base + rec_bytes);
   hipLaunchKernelGGL(k_store_job, dim3(1), dim3(64), 0, stream, plan.jobs[0], plan.d_jobs);
   ZK_LAUNCH_CHECK("msm job table");
