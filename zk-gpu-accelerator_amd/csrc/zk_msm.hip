@@ -48,8 +48,7 @@
 //              groups of windows side by side, all with the limb-sliced formulas of
 //              zk_curve_sliced.hpp, and the Jacobian result stays on the device (the
 //              reference combines windows on the CPU, msm_gpu.py:449 and
-//              msm.py:156-169). Terms, window totals and the final combination are one
-//              launch: the last workgroup to finish a window (or a job) goes on.
+//              msm.py:156-169).
 //
 // Steps 1 and 2 and the clearing of the buckets share the plan's first launch (k_prepare).
 //
@@ -1125,14 +1124,14 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
   if (threadIdx.x == 0) parts[blockIdx.x] = shj[0];
 }
 
-// The window sums U = sum T_s + sum_j 2^(j + log_seg) Q_j and the final combination, in one launch
-// (k_window_tail). Terms: four workgroups of four wavefronts per (job, window), one wavefront per
-// SIMD (the limb-sliced formulas keep a SIMD's issue busy with one wavefront, so a second one
-// there would only wait). Wavefront slot i of the window takes the terms i, i + 16, ...: a term is
-// the sum of its reduction row's chunk partials; term j < log_segs is Q_j doubled j + log_seg
-// times, the last two are the halves of sum T_s. The doublings of different terms run side by
-// side (a Horner pass over j would chain all of them with the additions); each workgroup sums its
-// wavefronts' terms, and the window's last workgroup adds the four partials.
+// Window sums U = sum T_s + sum_j 2^(j + log_seg) Q_j, in two launches. k_window_terms: four
+// workgroups of four wavefronts per (job, window), one wavefront per SIMD (the limb-sliced formulas
+// keep a SIMD's issue busy with one wavefront, so a second one there would only wait). Wavefront
+// slot i of the window takes the terms i, i + 16, ...: a term is the sum of its reduction row's
+// chunk partials; term j < log_segs is Q_j doubled j + log_seg times, the last two are the halves
+// of sum T_s. The doublings of different terms run side by side (a Horner pass over j would chain
+// all of them with the additions); each workgroup sums its wavefronts' terms, and k_window_total
+// sums a window's four partials.
 constexpr int kWinGroups = 4;
 constexpr int kWinWaves = 4;
 
@@ -1154,84 +1153,72 @@ __device__ __forceinline__ SlJac sl_workgroup_sum(JacRec* sh, const SlLane& c, c
   return sl_from_jac(load_jac_rec(&sh[0]));
 }
 
-// Final combination, by the workgroup of a job's last window: sum_w 2^(cw) U_w with the limb-sliced
-// formulas. The windows split into kFinalWaves contiguous groups, the top one the smallest;
-// wavefront g runs Horner over its group [lo, hi) from the top down, then doubles its sum c * lo
-// times, and the groups' results meet in a two-level tree. Every path keeps the (rw - 1) c
-// doublings a single Horner chain has (the top window's), but only the top group's additions and
-// the tree's stay on it: at 2^10 points (22 windows of 6 bits) 5 additions against 21.
-constexpr int kFinalWaves = 4;
-
-// Stages of k_window_tail follow each other without a launch between them: every workgroup counts
-// itself in its window's counter once its partial is stored; the last of a window's kWinGroups
-// workgroups adds the window's partials and counts the window in its job's counter; the last
-// window of a job runs the job's final combination. The counters start at zero (k_prepare clears
-// them with the buckets), and device-scope fences around the atomic increments make each stage's
-// stores visible to the workgroup that reads them, on whatever XCD it runs. Nothing waits: a
-// workgroup that is not last simply ends.
-__device__ __forceinline__ bool last_arrival(uint32_t* counter, uint32_t expected, uint32_t* ticket) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    *ticket = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  const bool last = *ticket == expected - 1;
-  if (last) __threadfence();
-  return last;
-}
-
-__global__ void __launch_bounds__(64 * kWinWaves) k_window_tail(const MsmJob* __restrict__ jobs, int njobs,
-                                                                int64_t total_windows, const JacRec* __restrict__ parts,
-                                                                JacRec* __restrict__ wparts, JacRec* __restrict__ wsum,
-                                                                uint32_t* __restrict__ counters) {
-  static_assert(kWinGroups == kWinWaves && kFinalWaves == kWinWaves, "one workgroup shape for the three stages");
+__global__ void __launch_bounds__(64 * kWinWaves) k_window_terms(const MsmJob* __restrict__ jobs, int njobs,
+                                                                 int64_t total_windows, const JacRec* __restrict__ parts,
+                                                                 JacRec* __restrict__ wparts) {
   __shared__ JacRec sh[kWinWaves];
-  __shared__ uint32_t ticket;
   const int64_t g = blockIdx.x / kWinGroups;
   const int group = (int)(blockIdx.x % kWinGroups);
   if (g >= total_windows) return;
-  const int jb = find_job<&MsmJob::win_base>(jobs, njobs, g);
+  const MsmJob& job = jobs[find_job<&MsmJob::win_base>(jobs, njobs, g)];
+  const int64_t win = g - job.win_base;
+  const int per = red_rows(job);
+  const JacRec* p = parts + job.part_base + win * per * job.chunks;
+  const FqConst f;
+  const SlLane c = sl_lane<FqConst>();
+  SlJac acc = sl_from_jac(jac_identity(f));
+  for (int t = group * kWinWaves + ((int)threadIdx.x >> 6); t < per; t += kWinGroups * kWinWaves) {
+    SlJac d = sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks));
+    for (int k = 1; k < job.chunks; ++k) d = sl_add<FqConst>(c, d, sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks + k)));
+    const int dbl = t < job.log_segs ? t + job.log_seg : 0;
+    for (int e = 0; e < dbl; ++e) d = sl_dbl<FqConst>(c, d);
+    acc = sl_add<FqConst>(c, acc, d);
+  }
+  acc = sl_workgroup_sum<kWinWaves>(sh, c, acc);
+  const Jac r = sl_to_jac<FqConst>(c, acc);
+  if (threadIdx.x == 0) store_jac_rec(wparts + blockIdx.x, r);
+}
+
+// One workgroup per (job, window): the sum of the window's kWinGroups partials.
+__global__ void __launch_bounds__(64 * kWinGroups) k_window_total(int64_t total_windows, const JacRec* __restrict__ wparts,
+                                                                  JacRec* __restrict__ wsum) {
+  __shared__ JacRec sh[kWinGroups];
+  const int64_t g = blockIdx.x;
+  if (g >= total_windows) return;
+  const SlLane c = sl_lane<FqConst>();
+  const SlJac mine = sl_from_jac(load_jac_rec(wparts + g * kWinGroups + ((int)threadIdx.x >> 6)));
+  const Jac r = sl_to_jac<FqConst>(c, sl_workgroup_sum<kWinGroups>(sh, c, mine));
+  if (threadIdx.x == 0) store_jac_rec(wsum + g, r);
+}
+
+// One workgroup per job: sum_w 2^(cw) U_w with the limb-sliced formulas. The windows split into
+// kFinalWaves contiguous groups, the top one the smallest; wavefront g runs Horner over its group
+// [lo, hi) from the top down, then doubles its sum c * lo times, and the groups' results meet in a
+// two-level tree. Every path keeps the (rw - 1) c doublings a single Horner chain has (the top
+// window's), but only the top group's additions and the tree's stay on it: at 2^10 points (22
+// windows of 6 bits) 5 additions against 21.
+constexpr int kFinalWaves = 4;
+
+__global__ void __launch_bounds__(64 * kFinalWaves) k_final(const MsmJob* __restrict__ jobs, int njobs,
+                                                            const JacRec* __restrict__ wsum) {
+  __shared__ JacRec sh[kFinalWaves];
+  const int jb = blockIdx.x;
+  if (jb >= njobs) return;
   const MsmJob& job = jobs[jb];
   const FqConst f;
   const SlLane c = sl_lane<FqConst>();
-  {  // this workgroup's terms of window g
-    const int64_t win = g - job.win_base;
-    const int per = red_rows(job);
-    const JacRec* p = parts + job.part_base + win * per * job.chunks;
-    SlJac acc = sl_from_jac(jac_identity(f));
-    for (int t = group * kWinWaves + ((int)threadIdx.x >> 6); t < per; t += kWinGroups * kWinWaves) {
-      SlJac d = sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks));
-      for (int k = 1; k < job.chunks; ++k) d = sl_add<FqConst>(c, d, sl_from_jac(load_jac_rec(p + (int64_t)t * job.chunks + k)));
-      const int dbl = t < job.log_segs ? t + job.log_seg : 0;
-      for (int e = 0; e < dbl; ++e) d = sl_dbl<FqConst>(c, d);
-      acc = sl_add<FqConst>(c, acc, d);
-    }
-    acc = sl_workgroup_sum<kWinWaves>(sh, c, acc);
-    const Jac r = sl_to_jac<FqConst>(c, acc);
-    if (threadIdx.x == 0) store_jac_rec(wparts + blockIdx.x, r);
+  const int per = (job.rw + kFinalWaves - 1) / kFinalWaves;
+  const int lo = min(job.rw, ((int)threadIdx.x >> 6) * per), hi = min(job.rw, lo + per);
+  SlJac acc = sl_from_jac(hi > lo ? load_jac_rec(wsum + job.win_base + hi - 1) : jac_identity(f));
+  for (int w = hi - 2; w >= lo; --w) {
+    for (int e = 0; e < job.c; ++e) acc = sl_dbl<FqConst>(c, acc);
+    acc = sl_add<FqConst>(c, acc, sl_from_jac(load_jac_rec(wsum + job.win_base + w)));
   }
-  if (!last_arrival(counters + g, kWinGroups, &ticket)) return;
-  {  // the window's total
-    const SlJac mine = sl_from_jac(load_jac_rec(wparts + g * kWinGroups + ((int)threadIdx.x >> 6)));
-    const Jac r = sl_to_jac<FqConst>(c, sl_workgroup_sum<kWinGroups>(sh, c, mine));
-    if (threadIdx.x == 0) store_jac_rec(wsum + g, r);
-  }
-  if (!last_arrival(counters + total_windows + jb, (uint32_t)job.rw, &ticket)) return;
-  {  // the job's final combination
-    const int per = (job.rw + kFinalWaves - 1) / kFinalWaves;
-    const int lo = min(job.rw, ((int)threadIdx.x >> 6) * per), hi = min(job.rw, lo + per);
-    SlJac acc = sl_from_jac(hi > lo ? load_jac_rec(wsum + job.win_base + hi - 1) : jac_identity(f));
-    for (int w = hi - 2; w >= lo; --w) {
-      for (int e = 0; e < job.c; ++e) acc = sl_dbl<FqConst>(c, acc);
-      acc = sl_add<FqConst>(c, acc, sl_from_jac(load_jac_rec(wsum + job.win_base + w)));
-    }
-    if (hi > lo)
-      for (int e = job.c * lo; e > 0; --e) acc = sl_dbl<FqConst>(c, acc);
-    Jac r = sl_to_jac<FqConst>(c, sl_workgroup_sum<kFinalWaves>(sh, c, acc));
-    if (fe_is_zero_lazy(f, r.z)) r = jac_identity(f);
-    if (threadIdx.x == 0) store_jac(f, job.out, r);
-  }
+  if (hi > lo)
+    for (int e = job.c * lo; e > 0; --e) acc = sl_dbl<FqConst>(c, acc);
+  Jac r = sl_to_jac<FqConst>(c, sl_workgroup_sum<kFinalWaves>(sh, c, acc));
+  if (fe_is_zero_lazy(f, r.z)) r = jac_identity(f);
+  if (threadIdx.x == 0) store_jac(f, job.out, r);
 }
 
 // Window table of an SRS: out[w * n + i] = 2^(c w) P_i as an affine record, w < windows (one
@@ -1330,7 +1317,6 @@ struct MsmPlan {
   RunSummary* runs[2] = {nullptr, nullptr};
   XyzzRec *tot = nullptr, *run = nullptr;
   JacRec *parts_buf = nullptr, *wparts = nullptr, *wsum = nullptr;
-  uint32_t* tail_counters = nullptr;  // after the buckets: one per window, then one per job
 };
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1448,7 +1434,7 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   L->chunk_base = take(4 * (size_t)(pkeys + 1));
   L->counts_b = take(4 * ((size_t)plan->max_chunks << kPartBits));
   L->counts_raw = take(4 * ((size_t)plan->max_chunks << kPartBits));
-  L->buckets = take(sizeof(BucketRec) * (size_t)keys + 4 * (size_t)(wins + (int64_t)jobs.size()));  // + tail counters
+  L->buckets = take(sizeof(BucketRec) * (size_t)keys);
   L->r0 = take(sizeof(RunSummary) * (size_t)max_slots);
   L->r1 = take(sizeof(RunSummary) * (size_t)max_slots1);
   L->tot = take(sizeof(XyzzRec) * (size_t)segs);
@@ -1479,7 +1465,6 @@ static void msm_plan_bind(MsmPlan* plan, const MsmLayout& L, void* mem) {
   plan->counts_raw = reinterpret_cast<uint32_t*>(base + L.counts_raw);
   plan->scan_ws = reinterpret_cast<uint32_t*>(base + L.scan);
   plan->buckets = reinterpret_cast<BucketRec*>(base + L.buckets);
-  plan->tail_counters = reinterpret_cast<uint32_t*>(plan->buckets + plan->nkeys);
   plan->runs[0] = reinterpret_cast<RunSummary*>(base + L.r0);
   plan->runs[1] = reinterpret_cast<RunSummary*>(base + L.r1);
   plan->tot = reinterpret_cast<XyzzRec*>(base + L.tot);
@@ -1525,12 +1510,11 @@ void msm_plan_destroy(MsmPlan* plan) {
   delete plan;
 }
 
-// Clears the plan's buckets (every launch starts from empty ones).
 // The plan's first launch: digits, the buckets cleared (a fill kernel, not a memset node, so that
 // graph replays clear them too), and the raw points converted when the plan has any.
 static int msm_plan_prepare(MsmPlan* plan, hipStream_t stream) {
   static_assert(sizeof(BucketRec) == 36 * 4, "a bucket record is 36 words");
-  const int64_t clear_words = plan->nkeys * 36 + plan->windows_total + plan->njobs;  // buckets and tail counters
+  const int64_t clear_words = plan->nkeys * 36;
   const int64_t grid = std::min<int64_t>(
       std::max<int64_t>({plan->blocks, (int64_t)grid_for(clear_words, 256), (int64_t)grid_for(plan->prep_n, 256), 1}),
       (int64_t)1 << 16);
@@ -1660,11 +1644,15 @@ static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream) {
     hipLaunchKernelGGL(k_reduce_bits, dim3((unsigned)plan->parts), dim3(256), 0, stream, plan->d_jobs, nj, plan->tot,
                        plan->run, plan->parts_buf);
     ZK_LAUNCH_CHECK("msm reduce bits");
-    hipLaunchKernelGGL(k_window_tail, dim3((unsigned)(plan->windows_total * kWinGroups)), dim3(64 * kWinWaves), 0, stream,
-                       plan->d_jobs, nj, plan->windows_total, plan->parts_buf, plan->wparts, plan->wsum,
-                       plan->tail_counters);
+    hipLaunchKernelGGL(k_window_terms, dim3((unsigned)(plan->windows_total * kWinGroups)), dim3(64 * kWinWaves), 0, stream,
+                       plan->d_jobs, nj, plan->windows_total, plan->parts_buf, plan->wparts);
+    ZK_LAUNCH_CHECK("msm window terms");
+    hipLaunchKernelGGL(k_window_total, dim3((unsigned)plan->windows_total), dim3(64 * kWinGroups), 0, stream,
+                       plan->windows_total, plan->wparts, plan->wsum);
     ZK_LAUNCH_CHECK("msm window sums");
   }
+  hipLaunchKernelGGL(k_final, dim3((unsigned)nj), dim3(64 * kFinalWaves), 0, stream, plan->d_jobs, nj, plan->wsum);
+  ZK_LAUNCH_CHECK("msm final");
   return 0;
 }
 
