@@ -225,6 +225,10 @@ def main() -> int:
             "units_source": "the plan's sorted-entry count (zk_msm_g1_entries on the timed inputs)",
             "counters_source": "profiles/k_accumulate_msm_2p20_counters.txt",
             "work_per_unit": "one mixed addition: 11 Fq mul x 136 32-bit mad; 72 bytes (key, point ref, affine point)",
+            "product_ceiling": {**_product_fields(entries * MADD_MADS / 162 / (acc_avg * 1e-3), gb),
+                                "products_per_unit": round(MADD_MADS / 162, 3),
+                                "note": "madd-2008-s as executed (1,467 multiply-adds = 9.06 products of 162) "
+                                        "against zk_product_peak"},
         },
         "cpu_baseline": cpu,
         **extras,
@@ -342,6 +346,32 @@ def _valu_peaks(gb, runs: int = 5) -> dict:
         _PEAKS.update(derived=derived, measured_max=max(samples), samples=[round(x, 3) for x in samples],
                       source=f"{cus.value} CUs x 64 lanes x {clk.value // 1000} MHz; max of {runs} zk_valu_peak runs")
     return _PEAKS
+
+
+def _product_peak(gb, runs: int = 3) -> float:
+    """Measured 9-limb Montgomery products per second (10^9) on every lane with no memory traffic
+    (zk_product_peak): the ceiling the field arithmetic sets. Maximum of `runs`, once per process."""
+    import ctypes
+
+    if "products" not in _PEAKS:
+        out = ctypes.c_double(0)
+        samples = []
+        for _ in range(runs):
+            gb.call("zk_product_peak", ctypes.byref(out))
+            samples.append(float(out.value))
+        _PEAKS["products"] = max(samples)
+    return _PEAKS["products"]
+
+
+# multiply-adds of one madd-2008-s in XYZZ coordinates: 6 products of 162 (two 81-term halves), two
+# squarings of 126, and one pair of products that share a reduction (3 halves, 243)
+MADD_MADS = 6 * 162 + 2 * 126 + 243
+
+
+def _product_fields(products_per_s: float, gb) -> dict:
+    peak = _product_peak(gb)
+    return {"measured_gproducts_s": round(peak, 2), "achieved_gproducts_s": round(products_per_s / 1e9, 2),
+            "frac": round(products_per_s / 1e9 / peak, 4)}
 
 
 def _peak_fields(achieved: float, peaks: dict) -> dict:
@@ -498,6 +528,9 @@ def _ntt_and_pipeline(args, gb, Fr, workloads, log, quotient: bool = True) -> di
                 "achieved": round(ntt_tmads, 4),
                 **_peak_fields(ntt_tmads, peaks),
                 "unit": "Tmad/s",
+                "product_ceiling": {**_product_fields(args.log_ntt * (n // 2) / (ntt_pass_ms * 1e-3), gb),
+                                    "products_per_unit": 1.0,
+                                    "note": "one 9-limb Montgomery product per butterfly against zk_product_peak"},
                 "work_per_unit": "one butterfly: 1 Fr mul x 136 32-bit mad; n/2 butterflies x log n stages",
                 "hbm": {"achieved": round(pass_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                         "frac": round(pass_gbps / HBM_PEAK_GBPS, 4), "bytes_per_pass": 64 * n},

@@ -87,6 +87,11 @@ int zk_prof_read(const char* name, double* total_ms, int64_t* count, int reset);
  * field product is built from) on the current GPU, in 10^12 per second: the VALU roofline
  * of the field, NTT and MSM kernels (no counterpart in the reference). */
 int zk_valu_peak(double* mads_per_second_t);
+/* Measured throughput of whole 9-limb Montgomery products of the Fq field (independent chains on
+ * every lane, no memory traffic), in 10^9 products per second: the ceiling the field arithmetic
+ * itself sets for the NTT and MSM kernels, with its instructions beside the multiply-adds (no
+ * counterpart in the reference). */
+int zk_product_peak(double* gproducts_per_s);
 /* Probe kernel: replaces the CuPy probe (gpu_backend.py:94-99). */
 int zk_selftest(void);
 

@@ -208,3 +208,16 @@ def test_device_arrays_are_views():
     assert rows.reshape(5, 2, 4).shape == (5, 2, 4)
     assert BN254_FR.to_ints(rows.copy()) == list(range(10))
     assert gpu_backend.asarray(rows.get()).tolist() == rows.tolist()
+
+
+def test_throughput_microbenchmarks_report_plausible_rates():
+    """zk_valu_peak and zk_product_peak (the bench line's VALU and product ceilings) run and report
+    rates an MI355X can reach: at most one v_mad_u64_u32 per lane per clock, and a 9-limb product
+    costs 162 of them, so products/s stay below the multiply-add rate / 162."""
+    import ctypes
+
+    mads, prods = ctypes.c_double(0), ctypes.c_double(0)
+    gpu_backend.call("zk_valu_peak", ctypes.byref(mads))
+    gpu_backend.call("zk_product_peak", ctypes.byref(prods))
+    assert 1.0 < mads.value < 60.0  # 10^12 multiply-adds per second
+    assert 10.0 < prods.value < mads.value * 1e3 / 162  # 10^9 products per second

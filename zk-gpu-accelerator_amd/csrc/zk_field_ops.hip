@@ -140,7 +140,55 @@ static int binary(int field_id, const uint64_t* a, int a_bcast, const uint64_t* 
 
 using namespace zk;
 
+// Product throughput: every thread runs kChains independent chains of 9-limb Montgomery products
+// (the serial column schedule the throughput kernels use), values kept below 2p. No memory traffic
+// beyond one store per thread: the rate is the ceiling the field arithmetic itself sets.
+constexpr int kPeakChains = 4;
+
+__global__ void __launch_bounds__(256) k_product_peak(uint32_t* out, int iters, uint32_t seed) {
+  const FqConst f;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  Fe a[kPeakChains], b;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    b.l[i] = (seed * 2654435761u + t * 40503u + (uint32_t)i * 97u) & (LMASK >> 2);
+#pragma unroll
+    for (int c = 0; c < kPeakChains; ++c) a[c].l[i] = (t * 2246822519u + (uint32_t)(c * 9 + i) * 3266489917u) & (LMASK >> 2);
+  }
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int c = 0; c < kPeakChains; ++c) a[c] = mont_raw_serial(f, a[c], b);
+  }
+  uint32_t x = 0;
+#pragma unroll
+  for (int c = 0; c < kPeakChains; ++c)
+#pragma unroll
+    for (int i = 0; i < NL; ++i) x ^= a[c].l[i];
+  out[t] = x;
+}
+
 extern "C" {
+
+int zk_product_peak(double* gproducts_per_s) {
+  const int blocks = device_cus() * 12, threads = 256, iters = 400;
+  uint32_t* out = nullptr;
+  ZK_HIP(hipMalloc(&out, (size_t)blocks * threads * 4));
+  hipEvent_t e0, e1;
+  ZK_HIP(hipEventCreate(&e0));
+  ZK_HIP(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_product_peak, dim3(blocks), dim3(threads), 0, 0, out, 4, 1u);  // warm up
+  ZK_HIP(hipEventRecord(e0, 0));
+  hipLaunchKernelGGL(k_product_peak, dim3(blocks), dim3(threads), 0, 0, out, iters, 7u);
+  ZK_HIP(hipEventRecord(e1, 0));
+  ZK_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  ZK_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(out);
+  *gproducts_per_s = (double)blocks * threads * iters * kPeakChains / (ms * 1e-3) / 1e9;
+  return 0;
+}
 
 int zk_field_register(const uint64_t modulus[4], int* field_id) {
   FieldRT f;

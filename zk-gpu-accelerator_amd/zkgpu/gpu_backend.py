@@ -88,6 +88,7 @@ SIGNATURES: dict[str, list[Any]] = {
     "zk_graph_launch": [_vp, _vp],
     "zk_graph_destroy": [_vp],
     "zk_valu_peak": [ctypes.POINTER(ctypes.c_double)],
+    "zk_product_peak": [ctypes.POINTER(ctypes.c_double)],
     "zk_msm_default_window": [_i64, ctypes.POINTER(_i32)],
     "zk_msm_set_glv": [_i32],
     "zk_msm_get_glv": [ctypes.POINTER(_i32)],
