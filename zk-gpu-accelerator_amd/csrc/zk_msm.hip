@@ -388,6 +388,10 @@ __global__ void __launch_bounds__(256) k_prepare(const MsmJob* __restrict__ jobs
 constexpr int kPartBits = 9;              // buckets per partition: 512
 constexpr int kPartMax = 1 << 11;         // partitions per window at most (c <= 20)
 constexpr int kChunkB = 8192;             // records per pass-B workgroup
+// Digits or records a thread loads before it updates the LDS counters or cursors: the loads of one
+// batch are in flight together (one load at a time left these passes waiting on memory latency).
+constexpr int kSortBatch = 8;
+constexpr int kPlaceThreads = 512;        // pass-B placement workgroup (two per CU, by its LDS)
 
 // Sort block geometry: (window, scalar tile) of one job.
 struct SortBlock {
@@ -419,9 +423,13 @@ __global__ void __launch_bounds__(256) k_sort_count(const MsmJob* __restrict__ j
   __syncthreads();
   const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.nv, lo + job.tile);
   const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.nv;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
-    const int32_t v = d[i];
-    if (v != 0) atomicAdd(&hist[((v < 0 ? -v : v) - 1) >> job.pbits], 1u);
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 256 * kSortBatch) {
+    int32_t v[kSortBatch];
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u) v[u] = i0 + 256 * u < hi ? d[i0 + 256 * u] : 0;
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u)
+      if (v[u] != 0) atomicAdd(&hist[((v[u] < 0 ? -v[u] : v[u]) - 1) >> job.pbits], 1u);
   }
   __syncthreads();
   uint32_t* out = counts + job.cnt_base + ((int64_t)sb.win * job.tiles + sb.tile) * np;
@@ -460,11 +468,18 @@ __global__ void __launch_bounds__(256) k_sort_columns(const MsmJob* __restrict__
 }
 
 // Pass A, placement: each digit's (bucket key, point ref) record goes to its partition's next
-// free position (LDS cursors per partition, no global atomics).
+// free position (LDS cursors per partition, no global atomics). A batch of 256 x kSortBatch digits
+// is first ordered by partition in LDS (a histogram, its scan, each record at its partition's
+// offset plus its rank), so that a partition's records of the batch leave as one run of
+// consecutive positions instead of as single 8-byte stores.
 __global__ void __launch_bounds__(256) k_sort_place(const MsmJob* __restrict__ jobs, int njobs,
                                                     const int32_t* __restrict__ digits, const uint32_t* __restrict__ counts,
                                                     const uint32_t* __restrict__ starts, uint2* __restrict__ staged) {
-  __shared__ uint32_t cur[kPartMax];
+  constexpr int kBatch = 256 * kSortBatch;
+  __shared__ uint32_t cur[kPartMax];  // the partition's next position in staged
+  __shared__ uint32_t at[kPartMax];   // the batch's count per partition, then its first slot in buf
+  __shared__ uint2 buf[kBatch];
+  __shared__ uint32_t wsum[4];
   const SortBlock sb = sort_block(jobs, njobs);
   const MsmJob& job = *sb.job;
   const int np = parts_per_window(job);
@@ -473,19 +488,70 @@ __global__ void __launch_bounds__(256) k_sort_place(const MsmJob* __restrict__ j
   const int64_t pkey0 = job.pkey_base + (job.table ? 0 : (int64_t)sb.win * np);
   const int64_t key0 = job.key_base + (job.table ? 0 : (int64_t)sb.win * buckets);
   for (int k = threadIdx.x; k < np; k += 256) cur[k] = starts[pkey0 + k] + off[k];
-  __syncthreads();
   const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.nv, lo + job.tile);
   const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.nv;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
-    const int32_t v = d[i];
-    if (v != 0) {
-      const int64_t b = (int64_t)(v < 0 ? -v : v) - 1;
-      const uint32_t pos = atomicAdd(&cur[b >> job.pbits], 1u);
+  const int per = (np + 255) / 256;  // partitions per thread in the scan (at most kPartMax / 256)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t b0 = lo; b0 < hi; b0 += kBatch) {
+    for (int k = threadIdx.x; k < np; k += 256) at[k] = 0;
+    int32_t v[kSortBatch];
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u) {
+      const int64_t i = b0 + threadIdx.x + 256 * u;
+      v[u] = i < hi ? d[i] : 0;
+    }
+    __syncthreads();
+    uint32_t rank[kSortBatch];
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u)
+      if (v[u] != 0) rank[u] = atomicAdd(&at[((v[u] < 0 ? -v[u] : v[u]) - 1) >> job.pbits], 1u);
+    __syncthreads();
+    // exclusive scan of at[0, np): per partitions per thread, then across the lanes and the waves
+    uint32_t mine = 0;
+    for (int e = 0; e < per; ++e) {
+      const int k = threadIdx.x * per + e;
+      mine += k < np ? at[k] : 0u;
+    }
+    uint32_t incl = mine;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+      const uint32_t o = __shfl_up(incl, s, 64);
+      if (lane >= s) incl += o;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - mine;
+    for (int w = 0; w < wave; ++w) run += wsum[w];
+    const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    for (int e = 0; e < per; ++e) {
+      const int k = threadIdx.x * per + e;
+      if (k < np) {
+        const uint32_t c = at[k];
+        at[k] = run;
+        run += c;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u) {
+      if (v[u] == 0) continue;
+      const int64_t i = b0 + threadIdx.x + 256 * u;
+      const int64_t b = (int64_t)(v[u] < 0 ? -v[u] : v[u]) - 1;
       // the point record: 2^(c w) P_i from the window table, P_i, or phi(P_i) for the k2 stream
       const int64_t rec = job.table ? job.table_base + (int64_t)sb.win * job.table_stride + i
                                     : (i < job.n ? i : job.phi_base + (i - job.n));
-      staged[pos] = make_uint2((uint32_t)(key0 + b), (uint32_t)rec | (v < 0 ? 0x80000000u : 0u));
+      buf[at[b >> job.pbits] + rank[u]] = make_uint2((uint32_t)(key0 + b), (uint32_t)rec | (v[u] < 0 ? 0x80000000u : 0u));
     }
+    __syncthreads();
+    // buf is in partition order: slot x of partition k goes to cur[k] plus its rank in the batch
+    for (int x = threadIdx.x; x < (int)total; x += 256) {
+      const uint2 r = buf[x];
+      const int k = (int)(((int64_t)r.x - key0) >> job.pbits);
+      staged[cur[k] + ((uint32_t)x - at[k])] = r;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < np; k += 256) cur[k] += (k + 1 < np ? at[k + 1] : total) - at[k];
+    __syncthreads();
   }
 }
 
@@ -563,6 +629,25 @@ __device__ __forceinline__ bool part_chunk(const MsmJob* __restrict__ jobs, int 
   return true;
 }
 
+// atomicAdd(&ctr[k], 1) for every active lane, returning the old value: the lanes whose key equals
+// the first active lane's are served by one atomic of their count (their ranks follow the lane
+// order), the others by one each. A run of one bucket, as a hot bucket's chunks are, would
+// otherwise queue 64 atomics on one LDS word per wavefront instruction.
+__device__ __forceinline__ uint32_t lds_claim(uint32_t* ctr, uint32_t k) {
+  const uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
+  const uint64_t same = __ballot(k == k0);
+  uint32_t pos;
+  if (k == k0) {
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
+    uint32_t base = 0;
+    if (rank == 0) base = atomicAdd(&ctr[k0], (uint32_t)__popcll(same));
+    pos = (uint32_t)__builtin_amdgcn_readlane((int)base, __ffsll((unsigned long long)same) - 1) + rank;
+  } else {
+    pos = atomicAdd(&ctr[k], 1u);
+  }
+  return pos;
+}
+
 // Pass B, counts: per (chunk, bucket of its partition).
 __global__ void __launch_bounds__(256) k_part_count(const MsmJob* __restrict__ jobs, int njobs,
                                                     const uint32_t* __restrict__ chunk_base, int64_t nparts,
@@ -575,7 +660,14 @@ __global__ void __launch_bounds__(256) k_part_count(const MsmJob* __restrict__ j
   const int nb = 1 << pc.pbits;
   for (int k = threadIdx.x; k < nb; k += 256) hist[k] = 0;
   __syncthreads();
-  for (int64_t x = pc.lo + threadIdx.x; x < pc.hi; x += 256) atomicAdd(&hist[staged[x].x - pc.bkey0], 1u);
+  for (int64_t x0 = pc.lo + threadIdx.x; x0 < pc.hi; x0 += 256 * kSortBatch) {
+    uint32_t k[kSortBatch];
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u) k[u] = x0 + 256 * u < pc.hi ? staged[x0 + 256 * u].x : 0u;
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u)
+      if (x0 + 256 * u < pc.hi) lds_claim(hist, (uint32_t)(k[u] - pc.bkey0));
+  }
   __syncthreads();
   uint32_t* out = counts + (int64_t)blockIdx.x * (1 << kPartBits);
   uint32_t* keep = raw + (int64_t)blockIdx.x * (1 << kPartBits);
@@ -605,11 +697,15 @@ __global__ void __launch_bounds__(256) k_part_columns(const MsmJob* __restrict__
     const int j = threadIdx.x * kPer + e;
     run[e] = 0;
     if (j < nb) {
-      for (int64_t g = g0; g < g1; ++g) {
-        uint32_t* c = counts + g * (1 << kPartBits) + j;
-        const uint32_t v = *c;
-        *c = run[e];
-        run[e] += v;
+      for (int64_t g = g0; g < g1; g += kSortBatch) {
+        uint32_t v[kSortBatch];
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) v[u] = g + u < g1 ? counts[(g + u) * (1 << kPartBits) + j] : 0u;
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) {
+          if (g + u < g1) counts[(g + u) * (1 << kPartBits) + j] = run[e];
+          run[e] += v[u];
+        }
       }
     }
   }
@@ -630,7 +726,14 @@ __global__ void __launch_bounds__(256) k_part_columns(const MsmJob* __restrict__
   for (int e = 0; e < kPer; ++e) {
     const int j = threadIdx.x * kPer + e;
     if (j < nb)
-      for (int64_t g = g0; g < g1; ++g) counts[g * (1 << kPartBits) + j] += base;
+      for (int64_t g = g0; g < g1; g += kSortBatch) {
+        uint32_t v[kSortBatch];
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) v[u] = g + u < g1 ? counts[(g + u) * (1 << kPartBits) + j] : 0u;
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u)
+          if (g + u < g1) counts[(g + u) * (1 << kPartBits) + j] = v[u] + base;
+      }
     base += run[e];
   }
 }
@@ -638,7 +741,7 @@ __global__ void __launch_bounds__(256) k_part_columns(const MsmJob* __restrict__
 // Pass B, placement: the chunk's records are sorted by bucket in LDS first (a histogram, its scan,
 // and LDS cursors), then leave in that order, so that a bucket's records of the chunk are
 // consecutive in memory and written as runs rather than one by one.
-__global__ void __launch_bounds__(256) k_part_place(const MsmJob* __restrict__ jobs, int njobs,
+__global__ void __launch_bounds__(kPlaceThreads) k_part_place(const MsmJob* __restrict__ jobs, int njobs,
                                                     const uint32_t* __restrict__ chunk_base, int64_t nparts,
                                                     const uint32_t* __restrict__ starts, const uint32_t* __restrict__ totals,
                                                     const uint2* __restrict__ staged, const uint32_t* __restrict__ counts,
@@ -647,14 +750,14 @@ __global__ void __launch_bounds__(256) k_part_place(const MsmJob* __restrict__ j
   __shared__ uint2 buf[kChunkB];
   __shared__ uint32_t first[kB];  // the bucket's first slot in buf
   __shared__ uint32_t cur[kB];
-  __shared__ uint32_t part[256];
+  __shared__ uint32_t part[kPlaceThreads];
   PartChunk pc;
   if (!part_chunk(jobs, njobs, chunk_base, nparts, starts, totals, blockIdx.x, &pc)) return;
   const int count = (int)(pc.hi - pc.lo);
-  // exclusive scan of the chunk's histogram (kept by the count pass): kB / 256 buckets per thread,
+  // exclusive scan of the chunk's histogram (kept by the count pass): kB / kPlaceThreads buckets per thread,
   // then over the threads
   const uint32_t* hist = raw + (int64_t)blockIdx.x * kB;
-  constexpr int kPer = kB > 256 ? kB / 256 : 1;
+  constexpr int kPer = kB > kPlaceThreads ? kB / kPlaceThreads : 1;
   uint32_t loc[kPer], sum = 0;
 #pragma unroll
   for (int e = 0; e < kPer; ++e) {
@@ -664,7 +767,7 @@ __global__ void __launch_bounds__(256) k_part_place(const MsmJob* __restrict__ j
   }
   part[threadIdx.x] = sum;
   __syncthreads();
-  for (int d = 1; d < 256; d <<= 1) {
+  for (int d = 1; d < kPlaceThreads; d <<= 1) {
     const uint32_t v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0u;
     __syncthreads();
     part[threadIdx.x] += v;
@@ -681,14 +784,18 @@ __global__ void __launch_bounds__(256) k_part_place(const MsmJob* __restrict__ j
     at += loc[e];
   }
   __syncthreads();
-  for (int x = threadIdx.x; x < count; x += 256) {
-    const uint2 r = staged[pc.lo + x];
-    buf[atomicAdd(&cur[r.x - pc.bkey0], 1u)] = r;
+  for (int x0 = threadIdx.x; x0 < count; x0 += kPlaceThreads * kSortBatch) {
+    uint2 r[kSortBatch];
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u) r[u] = x0 + kPlaceThreads * u < count ? staged[pc.lo + x0 + kPlaceThreads * u] : make_uint2(0u, 0u);
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u)
+      if (x0 + kPlaceThreads * u < count) buf[lds_claim(cur, (uint32_t)(r[u].x - pc.bkey0))] = r[u];
   }
   __syncthreads();
   // buf is now in bucket order: slot x of bucket j goes to the chunk's position for j plus its rank
   const uint32_t* base = counts + (int64_t)blockIdx.x * kB;
-  for (int x = threadIdx.x; x < count; x += 256) {
+  for (int x = threadIdx.x; x < count; x += kPlaceThreads) {
     const uint2 r = buf[x];
     const uint32_t j = r.x - (uint32_t)pc.bkey0;
     sorted[base[j] + (x - first[j])] = r;
@@ -1605,7 +1712,7 @@ static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream) {
     hipLaunchKernelGGL(k_part_columns, dim3((unsigned)plan->nparts), dim3(256), 0, stream, plan->d_jobs, nj, plan->chunk_base,
                        plan->nparts, plan->starts, plan->counts_b);
     ZK_LAUNCH_CHECK("msm sort partition columns");
-    hipLaunchKernelGGL(k_part_place, dim3((unsigned)plan->max_chunks), dim3(256), 0, stream, plan->d_jobs, nj, plan->chunk_base,
+    hipLaunchKernelGGL(k_part_place, dim3((unsigned)plan->max_chunks), dim3(kPlaceThreads), 0, stream, plan->d_jobs, nj, plan->chunk_base,
                        plan->nparts, plan->starts, plan->totals, plan->staged, plan->counts_b, plan->counts_raw,
                        plan->sorted);
     ZK_LAUNCH_CHECK("msm sort partition place");
