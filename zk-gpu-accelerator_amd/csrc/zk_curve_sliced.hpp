@@ -84,13 +84,16 @@ __device__ __forceinline__ int32_t sl_carry64(const SlLane& c, int64_t v) {
 }
 
 // a b 2^-261 (CIOS across the row). a, b: limbs 0..7 in [-2^6, 2^29 + 2^6], |a|, |b| below 2^258.
-// Every column stays below 2^62 in magnitude; the result is normalized.
+// A step's quotient m only has to be right mod 2^29, so only the last one is masked to 29 bits:
+// with the others below 2^32 the sum M of m_i 2^(29i) stays below 2^261 (1 + 2^-26), and a column
+// below 2^33 + 2^58.1 + 2^61 in magnitude. The result is normalized.
 template <class F>
 __device__ __forceinline__ int32_t sl_mul(const SlLane& c, int32_t a, int32_t b) {
   int64_t t = 0;
   sl_for<0, NL>([&](auto i) {
     t += (int64_t)sl_limb<decltype(i)::value>(a) * (int64_t)b;
-    const uint32_t m = ((uint32_t)sl_limb<0>((int32_t)t) * F::ninv) & LMASK;
+    uint32_t m = (uint32_t)sl_limb<0>((int32_t)t) * F::ninv;
+    if constexpr (decltype(i)::value == NL - 1) m &= LMASK;
     t = (int64_t)((uint64_t)t + (uint64_t)m * (uint32_t)c.p);  // column 0 is now 0 mod 2^29
     // shift down one limb: lane k takes lane k+1's low bits (row_shl:1) and keeps its carry
     t = (t >> LB) + (int64_t)(uint32_t)sl_dpp<0x101>((int32_t)((uint32_t)t & LMASK));
