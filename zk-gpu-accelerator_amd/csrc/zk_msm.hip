@@ -1095,24 +1095,32 @@ __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in
   }
 }
 
-// The `count` summaries of the previous level, 256 per workgroup, merged by a tree (empty
-// summaries are not copied).
+// The `count` summaries of the previous level, 256 per workgroup, merged by a tree. The non-empty
+// summaries are packed to the front in their order first (an empty summary holds no entries, so
+// the ones on either side of it are neighbours), and the tree is as wide as they are many: past the
+// boundary passes most summaries are empty, and a workgroup holding a few pieces of a hot bucket's
+// chain then merges them in a few levels instead of eight.
 __global__ void __launch_bounds__(kCombineLanes) k_combine(const RunSummary* __restrict__ in, int64_t count,
                                                            BucketRec* __restrict__ buckets, RunSummary* __restrict__ out,
                                                            int last_level) {
   __shared__ RunSummary sh[kCombineThreads];
-  if (threadIdx.x < kCombineThreads) {
-    const int64_t t = (int64_t)blockIdx.x * kCombineThreads + threadIdx.x;
-    const uint32_t flags = t < count ? in[t].flags : 0u;
-    if (flags) {
-      sh[threadIdx.x] = in[t];
-    } else {
-      sh[threadIdx.x].flags = 0;
-    }
+  __shared__ uint32_t wcount[kCombineThreads / 64];
+  const int64_t t = (int64_t)blockIdx.x * kCombineThreads + threadIdx.x;
+  const bool keep = threadIdx.x < kCombineThreads && t < count && in[t].flags != 0;
+  const uint64_t mask = __ballot(keep);
+  const int wave = (int)threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0 && wave < kCombineThreads / 64) wcount[wave] = (uint32_t)__popcll(mask);
+  __syncthreads();
+  int before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kCombineThreads / 64; ++w) {
+    before += w < wave ? (int)wcount[w] : 0;
+    total += (int)wcount[w];
   }
-  const int64_t valid = min((int64_t)kCombineThreads, count - (int64_t)blockIdx.x * kCombineThreads);
+  if (keep) sh[before + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u))] = in[t];
   int width = 1;
-  while (width < valid) width <<= 1;
+  while (width < total) width <<= 1;
+  if ((int)threadIdx.x >= total && (int)threadIdx.x < width) sh[threadIdx.x].flags = 0;
   merge_tree(sh, width, buckets);
   if (threadIdx.x == 0) {
     if (last_level) close_runs(&sh[0], buckets);
