@@ -6,12 +6,15 @@ strictly inside its range as finished buckets and leaves a summary of its first 
 (one run if its range holds a single key). A boundary pass (k_fixup in csrc/zk_msm.hip) then adds
 up every bucket cut into two fragments and writes it, and passes on only the fragments of longer
 chains (hot buckets through whole ranges) as reduced summaries; trees of summaries (k_combine,
-merge_runs) merge those level by level, and the last summary's open runs are written at the end.
+merge_runs) merge those level by level: each tree packs its non-empty summaries to the front first,
+and below the last level writes the runs of its result that do not continue into the summary just
+outside its range; the last summary's open runs are written at the end.
 
 This checks the scheme's invariant on random layouts (uniform keys, hot keys, ranges shorter and
 longer than buckets, empty ranges, trees of several widths, the boundary pass before the first
-and second levels as the library runs it): every bucket is written exactly once, with the sum of
-its entries. The GPU tests check the kernels themselves (tests/test_msm_gpu.py).
+level, with and without a second one before the second level, as the library runs it): every
+bucket is written exactly once, with the sum of its entries. The GPU tests check the kernels
+themselves (tests/test_msm_gpu.py).
 """
 
 import random
@@ -132,10 +135,43 @@ def merge(l, r, buckets):
     return l
 
 
-def tree_level(summ, buckets, width):
+def close_complete(res, summ, b, width, buckets):
+    """A tree's result below the last level: the runs that do not continue into the summary right
+    outside the tree's range are written, the rest goes on."""
+    if not res["flags"] & V:
+        return res
+    single = bool(res["flags"] & S)
+    last = res["fk"] if single else res["lk"]
+    cl = cr = False
+    if b > 0:
+        m = summ[b - 1]
+        cl = bool(m["flags"] & V) and (m["fk"] if m["flags"] & S else m["lk"]) == res["fk"]
+    if b + width < len(summ):
+        m = summ[b + width]
+        cr = bool(m["flags"] & V) and m["fk"] == last
+    res = dict(res)
+    if single:
+        if not cl and not cr:
+            write(buckets, res["fk"], res["fsum"])
+            res = empty()
+        return res
+    if not cl:
+        write(buckets, res["fk"], res["fsum"])
+    if not cr:
+        write(buckets, res["lk"], res["lsum"])
+    if not cl and not cr:
+        return empty()
+    if not cl:
+        return dict(flags=V | S, fk=res["lk"], lk=res["lk"], fsum=res["lsum"], lsum=0)
+    if not cr:
+        return dict(flags=V | S, fk=res["fk"], lk=res["fk"], fsum=res["fsum"], lsum=0)
+    return res
+
+
+def tree_level(summ, buckets, width, last_level):
     out = []
     for b in range(0, len(summ), width):
-        sh = summ[b : b + width]
+        sh = [x for x in summ[b : b + width] if x["flags"] & V]  # packed, in order
         w = 1
         while w < len(sh):
             w <<= 1
@@ -145,7 +181,7 @@ def tree_level(summ, buckets, width):
             for i in range(w // (2 * d)):
                 sh[2 * d * i] = merge(sh[2 * d * i], sh[2 * d * i + d], buckets)
             d <<= 1
-        out.append(sh[0])
+        out.append(sh[0] if last_level else close_complete(sh[0], summ, b, width, buckets))
     return out
 
 
@@ -166,11 +202,12 @@ def run_case(seed):
     width = rng.choice([2, 4, 8, 256])
     buckets = {}
     summ = thread_summaries(keys, vals, (m + threads - 1) // threads, threads, buckets)
+    second_pass = rng.random() < 0.5  # the library runs it when the threads outnumber the buckets
     level = 1
     while True:
-        if level <= 2:
+        if level == 1 or (level == 2 and second_pass):
             summ = boundary_pass(summ, buckets)
-        summ = tree_level(summ, buckets, width)
+        summ = tree_level(summ, buckets, width, len(summ) <= width)
         level += 1
         if len(summ) == 1:
             break

@@ -1122,10 +1122,50 @@ __global__ void __launch_bounds__(kCombineLanes) k_combine(const RunSummary* __r
   while (width < total) width <<= 1;
   if ((int)threadIdx.x >= total && (int)threadIdx.x < width) sh[threadIdx.x].flags = 0;
   merge_tree(sh, width, buckets);
-  if (threadIdx.x == 0) {
-    if (last_level) close_runs(&sh[0], buckets);
-    else out[blockIdx.x] = sh[0];
+  if (threadIdx.x != 0) return;
+  RunSummary* s = &sh[0];
+  if (last_level) {
+    close_runs(s, buckets);
+    return;
   }
+  // Runs that do not continue past the workgroup's range are complete: written now, so that the
+  // next level merges only what continues. A chain's pieces sit at consecutive, non-empty indices,
+  // so a run continues only into the summary right outside the range with the same key there.
+  if (s->flags & kValid) {
+    const int64_t base = (int64_t)blockIdx.x * kCombineThreads;
+    const bool single = s->flags & kSingle;
+    const uint32_t last = single ? s->fk : s->lk;
+    bool cl = false, cr = false;
+    if (base > 0) {
+      const uint4 m = *reinterpret_cast<const uint4*>(in + base - 1);  // fk, lk, flags of the left neighbour
+      cl = (m.z & kValid) && ((m.z & kSingle) ? m.x : m.y) == s->fk;
+    }
+    if (base + kCombineThreads < count) {
+      const uint4 m = *reinterpret_cast<const uint4*>(in + base + kCombineThreads);
+      cr = (m.z & kValid) && m.x == last;
+    }
+    if (single) {
+      if (!cl && !cr) {
+        store_bucket(buckets + s->fk, load_jac_rec(&s->fsum));
+        s->flags = 0;
+      }
+    } else {
+      if (!cl) store_bucket(buckets + s->fk, load_jac_rec(&s->fsum));
+      if (!cr) store_bucket(buckets + s->lk, load_jac_rec(&s->lsum));
+      if (!cl && !cr) {
+        s->flags = 0;
+      } else if (!cl) {
+        // only the last run goes on, as a single run
+        s->fk = s->lk;
+        s->fsum = s->lsum;
+        s->flags = kValid | kSingle;
+      } else if (!cr) {
+        s->lk = s->fk;
+        s->flags = kValid | kSingle;
+      }
+    }
+  }
+  out[blockIdx.x] = *s;
 }
 
 // Leaf of the bucket reduction: segment s of a window covers buckets [sL, sL + L)
@@ -1733,12 +1773,19 @@ static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream) {
   {
     // every pass writes the other summary buffer: check the count against its capacity first
     const int64_t cap[2] = {plan->max_slots, plan->max_slots1};
+    // A second boundary pass, before the second tree level, pays when the threads outnumber the
+    // buckets: ranges are then shorter than a typical bucket, most buckets reach the trees, and the
+    // first level leaves many two-piece chains across its workgroups' edges, which the pass adds up
+    // side by side. With longer ranges only the hot buckets' chains remain there, and the trees'
+    // own closing of complete runs leaves the pass nothing to do.
+    const bool second_pass = plan->level_max[0] > plan->nkeys;
     int cur = 0;
     for (int l = 1; l < levels; ++l) {
-      if ((l <= 2 && plan->level_max[(size_t)l - 1] > cap[cur ^ 1]) || plan->level_max[(size_t)l] > cap[(l <= 2 ? cur : cur ^ 1)])
+      const bool pass = l == 1 || (l == 2 && second_pass);
+      if ((pass && plan->level_max[(size_t)l - 1] > cap[cur ^ 1]) || plan->level_max[(size_t)l] > cap[pass ? cur : cur ^ 1])
         return fail("internal: MSM run-summary buffer smaller than a level");
-      if (l <= 2) {
-        // before the first and second tree levels: resolve the summaries' boundaries in one pass
+      if (pass) {
+        // resolve the summaries' boundaries in one pass
         hipLaunchKernelGGL(k_fixup, dim3(grid_for(plan->level_max[l - 1], 256)), dim3(256), 0, stream,
                            plan->runs[cur], plan->level_max[l - 1], plan->buckets, plan->runs[cur ^ 1]);
         ZK_LAUNCH_CHECK("msm boundaries");
