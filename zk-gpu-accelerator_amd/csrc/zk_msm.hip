@@ -916,9 +916,10 @@ constexpr int kCombineThreads = 256;  // summaries per combine workgroup
 // Threads of a combine workgroup: the tree's upper levels pick their mode from the workgroup's rows
 // and wavefronts, so 512 threads (32 rows, 8 wavefronts) run the level of 32 merges one per row and
 // the last four one per wavefront (limb-sliced) instead of one per thread and per row. At 228
-// registers a CU then holds one such workgroup instead of two, so levels with more workgroups than
-// CUs keep 256 threads (1024 threads would cap the registers at 128, below the thread-serial
-// addition's).
+// registers a CU holds one such workgroup, but past the boundary pass most workgroups of a level
+// hold no summary and end at once, so even a level with more workgroups than CUs waits only on
+// the few that merge a hot bucket's pieces (1024 threads would cap the registers at 128, below the
+// thread-serial addition's).
 constexpr int kCombineLanes = 512;
 
 // Level 0, a persistent grid: the m sorted entries are split evenly, ceil(m / threads) consecutive
@@ -1791,7 +1792,7 @@ static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream) {
         ZK_LAUNCH_CHECK("msm boundaries");
         cur ^= 1;
       }
-      const int lanes = plan->level_max[(size_t)l] <= device_cus() ? kCombineLanes : kCombineThreads;
+      const int lanes = kCombineLanes;
       hipLaunchKernelGGL(k_combine, dim3((unsigned)plan->level_max[(size_t)l]), dim3(lanes), 0, stream,
                          plan->runs[cur], plan->level_max[(size_t)l - 1], plan->buckets, plan->runs[cur ^ 1],
                          l + 1 == levels ? 1 : 0);
