@@ -35,4 +35,17 @@ int zkt_digits(const uint32_t* v, int c, int windows, int32_t* out) {
   return (int)carry;
 }
 
+// The digits of a GLV half as the digit kernel stores them: with `neg`, ties go negative below the
+// top window and every digit is negated (the half is -v).
+int zkt_half_digits(const uint32_t* v, int c, int windows, int neg, int32_t* out) {
+  uint32_t w[8];
+  for (int i = 0; i < 8; ++i) w[i] = v[i];
+  uint32_t carry = 0;
+  for (int win = 0; win < windows; ++win) {
+    const int d = zk::signed_digit(w, win, c, carry, neg && win + 1 < windows);
+    out[win] = neg ? -d : d;
+  }
+  return (int)carry;
+}
+
 }  // extern "C"

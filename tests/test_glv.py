@@ -115,3 +115,33 @@ def test_signed_digits_reconstruct(lib, c):
             assert carry == 0
             assert all(-(1 << (c - 1)) <= x <= (1 << (c - 1)) for x in d)
             assert sum(x << (c * i) for i, x in enumerate(d)) == v
+
+
+@pytest.mark.parametrize("c", [5, 13, 15, 16])
+def test_folded_half_digits_fit_16_bit_codes(lib, c):
+    """A GLV half's digits as stored for the sort (sign folded in, a negative half's ties negative
+    below the top window): they rebuild the half and lie in [-(2^(c-1) - 1), 2^(c-1)], so that
+    (|d| - 1) with a sign bit never sets all 16 bits of a code for c <= 16."""
+    rng = random.Random(100 + c)
+    out = (ctypes.c_int32 * 300)()
+    windows = 127 // c + 1
+    half = 1 << (c - 1)
+    # the halves the split gives (|k_i| < 2^126.97), and the window patterns with ties everywhere
+    ties = sum(half << (c * i) for i in range(windows - 1))
+    vals = [0, 1, ties, ties - 1, ties + 1, (1 << 126) + ties]
+    vals += [abs(x) for k in [rng.randrange(R) for _ in range(400)] for x in split(lib, k)]
+    for v in vals:
+        for neg in (0, 1):
+            carry = lib.zkt_half_digits(words(v), c, windows, neg, out)
+            d = [out[i] for i in range(windows)]
+            assert carry == 0, (v, neg)
+            assert sum(x << (c * i) for i, x in enumerate(d)) == (-v if neg else v)
+            assert all(-(half - 1) <= x <= half for x in d), (v, neg, d)
+
+
+def test_top_16_bit_window_of_a_half_stays_below_its_tie():
+    """The split's bound 1.125 (|a1| + |a2|) puts a half's top 16-bit window (bits 112..) at most
+    at 32055, so with its carry in it never reaches 2^15, the tie the 16-bit codes cannot fold."""
+    c = glv_struct()
+    for a, b in ((c["a1"], c["a2"]), (c["b1n"], c["b2"])):
+        assert ((9 * (a + b)) // 8 >> 112) + 1 < 1 << 15

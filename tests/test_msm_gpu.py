@@ -235,6 +235,22 @@ def test_reduction_segment_lengths(log_seg):
         gpu_backend.call("zk_msm_set_segment", old.value)
 
 
+@pytest.mark.parametrize("w", [16, 17, 20])
+def test_windows_past_16_bits(w):
+    """Windows up to 16 bits store 16-bit digit codes for the sort, wider ones (the device entry
+    point allows 20) 32-bit codes: both widths at their edges, with the largest digits a scalar can
+    give in the top window (2^127 - 1 and r - 1 split to halves with every window digit set)."""
+    n = 3000
+    keys = workloads.srs_keys(n, 0x5EED12)
+    scalars = workloads.witness_scalars(n, 0x5EED13)
+    scalars[:4] = [R - 1, 2**127 - 1, 0, 2**126 + 2**110 - 1]
+    pts = workloads.srs_points(ints_to_limbs(keys))
+    e = workloads.closed_form_scalar(keys, scalars)
+    out = gpu_backend.empty((3, 4))
+    msm_gpu.msm_g1_jacobian(pts, gpu_backend.asarray(ints_to_limbs(scalars)), out, window_size=w)
+    assert msm_gpu._result(out).affine() == (G * e).affine()
+
+
 @pytest.mark.parametrize("chunk", [0, 2, 5])
 def test_one_bucket_spans_every_range(chunk):
     """Scalars that send (almost) every entry to one bucket: the bucket's run crosses every range

@@ -334,11 +334,30 @@ __device__ __forceinline__ int find_job(const MsmJob* jobs, int njobs, int64_t x
   return lo;
 }
 
-// Signed digits of every scalar: digits[dig_base + w * nv + v] = d (0 for none), v = i for the
+// A signed digit d as the sort reads it: (|d| - 1) with the sign in the top bit, every bit set
+// for d = 0. Stored digits lie in [-(2^(c-1) - 1), 2^(c-1)] (a GLV half's sign folded in, its ties
+// chosen for that), so no nonzero digit has every bit set, and 16-bit codes hold windows of up to
+// 16 bits (a plan with wider ones stores 32-bit codes): the digit array is written once and read
+// twice, and at 2^20 points 16-bit codes move 100 MB less.
+template <class D>
+__device__ __forceinline__ D digit_code(int d) {
+  if (d == 0) return (D)~(D)0;
+  const uint32_t mag = (uint32_t)(d < 0 ? -d : d) - 1u;
+  return (D)(mag | (d < 0 ? 1u << (8 * sizeof(D) - 1) : 0u));
+}
+template <class D>
+__device__ __forceinline__ bool digit_any(D v) { return v != (D)~(D)0; }
+template <class D>
+__device__ __forceinline__ int64_t digit_bucket(D v) { return (int64_t)((uint32_t)v & ((1u << (8 * sizeof(D) - 1)) - 1u)); }
+template <class D>
+__device__ __forceinline__ bool digit_neg(D v) { return (v >> (8 * sizeof(D) - 1)) & 1; }
+
+// Signed digits of every scalar: digits[dig_base + w * nv + v] = the code of d, v = i for the
 // scalar itself, or with the GLV split v = i for k1 (point P_i) and v = n + i for k2 (phi(P_i)),
 // the sign of k1 / k2 folded into the digits. Block b of the digit grid (256 scalars).
+template <class D>
 __device__ __forceinline__ void signed_digits_block(const MsmJob* __restrict__ jobs, int njobs, int64_t b,
-                                                    int32_t* __restrict__ digits) {
+                                                    D* __restrict__ digits) {
   const MsmJob& job = jobs[find_job<&MsmJob::block_base>(jobs, njobs, b)];
   const int64_t i = (b - job.block_base) * 256 + threadIdx.x;
   if (i >= job.n) return;
@@ -349,10 +368,10 @@ __device__ __forceinline__ void signed_digits_block(const MsmJob* __restrict__ j
     w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w; w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
   }
   reduce_mod_r(w);
-  int32_t* d = digits + job.dig_base;
+  D* d = digits + job.dig_base;
   if (!job.glv) {
     uint32_t carry = 0;
-    for (int win = 0; win < job.windows; ++win) d[(int64_t)win * job.nv + i] = signed_digit(w, win, job.c, carry);
+    for (int win = 0; win < job.windows; ++win) d[(int64_t)win * job.nv + i] = digit_code<D>(signed_digit(w, win, job.c, carry));
     return;
   }
   uint32_t m1[8], m2[8];
@@ -360,17 +379,22 @@ __device__ __forceinline__ void signed_digits_block(const MsmJob* __restrict__ j
   glv_split(w, m1, neg1, m2, neg2);
   uint32_t c1 = 0, c2 = 0;
   for (int win = 0; win < job.windows; ++win) {
-    const int d1 = signed_digit(m1, win, job.c, c1), d2 = signed_digit(m2, win, job.c, c2);
-    d[(int64_t)win * job.nv + i] = neg1 ? -d1 : d1;
-    d[(int64_t)win * job.nv + job.n + i] = neg2 ? -d2 : d2;
+    // a negative half's ties go negative below the top window, so that its folded digits stay in
+    // [-(2^(c-1) - 1), 2^(c-1)] like the positive half's (the top window of a half below
+    // 2^126.97 never reaches 2^(c-1) for c = 16, the widest window with 16-bit codes)
+    const bool top = win + 1 == job.windows;
+    const int d1 = signed_digit(m1, win, job.c, c1, neg1 && !top), d2 = signed_digit(m2, win, job.c, c2, neg2 && !top);
+    d[(int64_t)win * job.nv + i] = digit_code<D>(neg1 ? -d1 : d1);
+    d[(int64_t)win * job.nv + job.n + i] = digit_code<D>(neg2 ? -d2 : d2);
   }
 }
 
 // The first launch of a plan, three independent jobs in one grid: the signed digits (blocks below
 // `digit_blocks`), the bucket records cleared to zero (one 32-bit word per thread), and, for a plan
 // over raw points, the point records (one point per thread). Grid-stride, so any grid covers them.
+template <class D>
 __global__ void __launch_bounds__(256) k_prepare(const MsmJob* __restrict__ jobs, int njobs, int64_t digit_blocks,
-                                                 int32_t* __restrict__ digits, uint32_t* __restrict__ clear,
+                                                 D* __restrict__ digits, uint32_t* __restrict__ clear,
                                                  int64_t clear_words, const uint64_t* __restrict__ pts, int64_t npts,
                                                  AffRec* __restrict__ out, AffRec* __restrict__ phi) {
   const int64_t threads = (int64_t)gridDim.x * 256;
@@ -413,8 +437,9 @@ __device__ __forceinline__ int parts_per_window(const MsmJob& job) { return 1 <<
 
 // Pass A, counts: per (partition, tile), the nonzero digits of one tile of one window. Counts are
 // laid out tile-major (window w, tile t: a row of the window's partitions).
+template <class D>
 __global__ void __launch_bounds__(256) k_sort_count(const MsmJob* __restrict__ jobs, int njobs,
-                                                    const int32_t* __restrict__ digits, uint32_t* __restrict__ counts) {
+                                                    const D* __restrict__ digits, uint32_t* __restrict__ counts) {
   __shared__ uint32_t hist[kPartMax];
   const SortBlock sb = sort_block(jobs, njobs);
   const MsmJob& job = *sb.job;
@@ -422,14 +447,14 @@ __global__ void __launch_bounds__(256) k_sort_count(const MsmJob* __restrict__ j
   for (int k = threadIdx.x; k < np; k += 256) hist[k] = 0;
   __syncthreads();
   const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.nv, lo + job.tile);
-  const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.nv;
+  const D* d = digits + job.dig_base + (int64_t)sb.win * job.nv;
   for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 256 * kSortBatch) {
-    int32_t v[kSortBatch];
+    D v[kSortBatch];
 #pragma unroll
-    for (int u = 0; u < kSortBatch; ++u) v[u] = i0 + 256 * u < hi ? d[i0 + 256 * u] : 0;
+    for (int u = 0; u < kSortBatch; ++u) v[u] = i0 + 256 * u < hi ? d[i0 + 256 * u] : (D)~(D)0;
 #pragma unroll
     for (int u = 0; u < kSortBatch; ++u)
-      if (v[u] != 0) atomicAdd(&hist[((v[u] < 0 ? -v[u] : v[u]) - 1) >> job.pbits], 1u);
+      if (digit_any(v[u])) atomicAdd(&hist[digit_bucket(v[u]) >> job.pbits], 1u);
   }
   __syncthreads();
   uint32_t* out = counts + job.cnt_base + ((int64_t)sb.win * job.tiles + sb.tile) * np;
@@ -472,8 +497,9 @@ __global__ void __launch_bounds__(256) k_sort_columns(const MsmJob* __restrict__
 // is first ordered by partition in LDS (a histogram, its scan, each record at its partition's
 // offset plus its rank), so that a partition's records of the batch leave as one run of
 // consecutive positions instead of as single 8-byte stores.
+template <class D>
 __global__ void __launch_bounds__(256) k_sort_place(const MsmJob* __restrict__ jobs, int njobs,
-                                                    const int32_t* __restrict__ digits, const uint32_t* __restrict__ counts,
+                                                    const D* __restrict__ digits, const uint32_t* __restrict__ counts,
                                                     const uint32_t* __restrict__ starts, uint2* __restrict__ staged) {
   constexpr int kBatch = 256 * kSortBatch;
   __shared__ uint32_t cur[kPartMax];  // the partition's next position in staged
@@ -489,22 +515,22 @@ __global__ void __launch_bounds__(256) k_sort_place(const MsmJob* __restrict__ j
   const int64_t key0 = job.key_base + (job.table ? 0 : (int64_t)sb.win * buckets);
   for (int k = threadIdx.x; k < np; k += 256) cur[k] = starts[pkey0 + k] + off[k];
   const int64_t lo = (int64_t)sb.tile * job.tile, hi = min(job.nv, lo + job.tile);
-  const int32_t* d = digits + job.dig_base + (int64_t)sb.win * job.nv;
+  const D* d = digits + job.dig_base + (int64_t)sb.win * job.nv;
   const int per = (np + 255) / 256;  // partitions per thread in the scan (at most kPartMax / 256)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int64_t b0 = lo; b0 < hi; b0 += kBatch) {
     for (int k = threadIdx.x; k < np; k += 256) at[k] = 0;
-    int32_t v[kSortBatch];
+    D v[kSortBatch];
 #pragma unroll
     for (int u = 0; u < kSortBatch; ++u) {
       const int64_t i = b0 + threadIdx.x + 256 * u;
-      v[u] = i < hi ? d[i] : 0;
+      v[u] = i < hi ? d[i] : (D)~(D)0;
     }
     __syncthreads();
     uint32_t rank[kSortBatch];
 #pragma unroll
     for (int u = 0; u < kSortBatch; ++u)
-      if (v[u] != 0) rank[u] = atomicAdd(&at[((v[u] < 0 ? -v[u] : v[u]) - 1) >> job.pbits], 1u);
+      if (digit_any(v[u])) rank[u] = atomicAdd(&at[digit_bucket(v[u]) >> job.pbits], 1u);
     __syncthreads();
     // exclusive scan of at[0, np): per partitions per thread, then across the lanes and the waves
     uint32_t mine = 0;
@@ -534,13 +560,13 @@ __global__ void __launch_bounds__(256) k_sort_place(const MsmJob* __restrict__ j
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kSortBatch; ++u) {
-      if (v[u] == 0) continue;
+      if (!digit_any(v[u])) continue;
       const int64_t i = b0 + threadIdx.x + 256 * u;
-      const int64_t b = (int64_t)(v[u] < 0 ? -v[u] : v[u]) - 1;
+      const int64_t b = digit_bucket(v[u]);
       // the point record: 2^(c w) P_i from the window table, P_i, or phi(P_i) for the k2 stream
       const int64_t rec = job.table ? job.table_base + (int64_t)sb.win * job.table_stride + i
                                     : (i < job.n ? i : job.phi_base + (i - job.n));
-      buf[at[b >> job.pbits] + rank[u]] = make_uint2((uint32_t)(key0 + b), (uint32_t)rec | (v[u] < 0 ? 0x80000000u : 0u));
+      buf[at[b >> job.pbits] + rank[u]] = make_uint2((uint32_t)(key0 + b), (uint32_t)rec | (digit_neg(v[u]) ? 0x80000000u : 0u));
     }
     __syncthreads();
     // buf is in partition order: slot x of partition k goes to cur[k] plus its rank in the batch
@@ -1461,7 +1487,8 @@ struct MsmPlan {
   int64_t prep_n = 0;
   AffRec* prep_out = nullptr;
   bool prep_phi = false;
-  int32_t* digits = nullptr;
+  void* digits = nullptr;
+  bool d16 = true;  // 16-bit digit codes (every job's window at most 16 bits), else 32-bit
   const AffRec* points = nullptr;
   // device buffers
   void* mem = nullptr;
@@ -1549,6 +1576,8 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   grid0 = std::max<int64_t>(1, grid0);
   plan->jobs = jobs;
   plan->njobs = (int)jobs.size();
+  plan->d16 = true;
+  for (const auto& j : jobs) plan->d16 = plan->d16 && j.c <= 16;
   plan->blocks = blocks;
   plan->nkeys = keys;
   plan->nparts = pkeys;
@@ -1611,7 +1640,7 @@ static void msm_plan_bind(MsmPlan* plan, const MsmLayout& L, void* mem) {
   plan->counts = reinterpret_cast<uint32_t*>(base + L.counts);
   plan->totals = reinterpret_cast<uint32_t*>(base + L.totals);
   plan->starts = reinterpret_cast<uint32_t*>(base + L.starts);
-  plan->digits = reinterpret_cast<int32_t*>(base + L.digits);
+  plan->digits = base + L.digits;
   plan->m_dev = reinterpret_cast<uint32_t*>(base + L.m);
   plan->staged = reinterpret_cast<uint2*>(base + L.staged);
   plan->sorted = reinterpret_cast<uint2*>(base + L.sorted);
@@ -1674,9 +1703,16 @@ static int msm_plan_prepare(MsmPlan* plan, hipStream_t stream) {
   const int64_t grid = std::min<int64_t>(
       std::max<int64_t>({plan->blocks, (int64_t)grid_for(clear_words, 256), (int64_t)grid_for(plan->prep_n, 256), 1}),
       (int64_t)1 << 16);
-  hipLaunchKernelGGL(k_prepare, dim3((unsigned)grid), dim3(256), 0, stream, plan->d_jobs, plan->njobs, plan->blocks,
-                     plan->digits, reinterpret_cast<uint32_t*>(plan->buckets), clear_words, plan->prep_src,
-                     plan->prep_n, plan->prep_out, plan->prep_phi ? plan->prep_out + plan->prep_n : nullptr);
+  AffRec* phi = plan->prep_phi ? plan->prep_out + plan->prep_n : nullptr;
+  uint32_t* clear = reinterpret_cast<uint32_t*>(plan->buckets);
+  if (plan->d16)
+    hipLaunchKernelGGL(k_prepare<uint16_t>, dim3((unsigned)grid), dim3(256), 0, stream, plan->d_jobs, plan->njobs,
+                       plan->blocks, static_cast<uint16_t*>(plan->digits), clear, clear_words, plan->prep_src,
+                       plan->prep_n, plan->prep_out, phi);
+  else
+    hipLaunchKernelGGL(k_prepare<uint32_t>, dim3((unsigned)grid), dim3(256), 0, stream, plan->d_jobs, plan->njobs,
+                       plan->blocks, static_cast<uint32_t*>(plan->digits), clear, clear_words, plan->prep_src,
+                       plan->prep_n, plan->prep_out, phi);
   ZK_LAUNCH_CHECK("msm digits");
   return 0;
 }
@@ -1729,8 +1765,12 @@ static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream) {
   const int nj = plan->njobs;
   if (int rc = msm_plan_prepare(plan, stream)) return rc;
   {
-    hipLaunchKernelGGL(k_sort_count, dim3((unsigned)plan->sort_blocks), dim3(256), 0, stream, plan->d_jobs, nj,
-                       plan->digits, plan->counts);
+    if (plan->d16)
+      hipLaunchKernelGGL(k_sort_count<uint16_t>, dim3((unsigned)plan->sort_blocks), dim3(256), 0, stream, plan->d_jobs, nj,
+                         static_cast<const uint16_t*>(plan->digits), plan->counts);
+    else
+      hipLaunchKernelGGL(k_sort_count<uint32_t>, dim3((unsigned)plan->sort_blocks), dim3(256), 0, stream, plan->d_jobs, nj,
+                         static_cast<const uint32_t*>(plan->digits), plan->counts);
     ZK_LAUNCH_CHECK("msm sort count");
     hipLaunchKernelGGL(k_sort_columns, dim3(grid_for(plan->nparts * 64, 256)), dim3(256), 0, stream, plan->d_jobs, nj,
                        plan->nparts, plan->counts, plan->totals);
@@ -1743,8 +1783,12 @@ static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream) {
     } else if (int rc = scan_u32_ws(plan->totals, plan->starts, plan->nparts, plan->m_dev, plan->scan_ws, stream)) {
       return rc;
     }
-    hipLaunchKernelGGL(k_sort_place, dim3((unsigned)plan->sort_blocks), dim3(256), 0, stream, plan->d_jobs, nj,
-                       plan->digits, plan->counts, plan->starts, plan->staged);
+    if (plan->d16)
+      hipLaunchKernelGGL(k_sort_place<uint16_t>, dim3((unsigned)plan->sort_blocks), dim3(256), 0, stream, plan->d_jobs, nj,
+                         static_cast<const uint16_t*>(plan->digits), plan->counts, plan->starts, plan->staged);
+    else
+      hipLaunchKernelGGL(k_sort_place<uint32_t>, dim3((unsigned)plan->sort_blocks), dim3(256), 0, stream, plan->d_jobs, nj,
+                         static_cast<const uint32_t*>(plan->digits), plan->counts, plan->starts, plan->staged);
     ZK_LAUNCH_CHECK("msm sort place");
     // pass B: every partition sorted by bucket, chunk by chunk
     if (!few) {

@@ -50,10 +50,13 @@ ZK_HD uint32_t raw_window(const uint32_t (&w)[8], int bit, int c) {
 
 // Signed digit of window `win` given the carry in, in [-2^(c-1), 2^(c-1)]; updates the carry.
 // Over windows 0..floor(B/c) the digits of any value below 2^B satisfy sum d_w 2^(cw) = value.
-ZK_HD int signed_digit(const uint32_t (&w)[8], int win, int c, uint32_t& carry) {
+// A window of exactly 2^(c-1) gives +2^(c-1), or with ties_negative -2^(c-1) and a carry (either
+// choice in any window below the top one leaves the sum unchanged), so that one window's digits
+// lie in [-(2^(c-1) - 1), 2^(c-1)], the other's in [-2^(c-1), 2^(c-1) - 1].
+ZK_HD int signed_digit(const uint32_t (&w)[8], int win, int c, uint32_t& carry, bool ties_negative = false) {
   const uint32_t raw = raw_window(w, win * c, c) + carry;
   const uint32_t half = 1u << (c - 1);
-  if (raw > half) {
+  if (raw > half || (ties_negative && raw == half)) {
     carry = 1;
     return (int)raw - (int)(1u << c);
   }
