@@ -938,7 +938,11 @@ __device__ __forceinline__ void close_runs(const RunSummary* sm, BucketRec* __re
 }
 
 constexpr int kAccThreads = 64;      // one wave per accumulation workgroup
-constexpr int kCombineThreads = 256;  // summaries per combine workgroup
+// Summaries per combine workgroup, chosen per plan (MsmPlan::group): 64 lets the tree's widest
+// level run one merge per 16-lane row (32 rows) instead of one per thread, which pays when the
+// level's workgroups are few or mostly empty (a hot bucket's chain through long ranges); 256 keeps
+// the workgroups few when most buckets reach the trees (ranges shorter than a bucket).
+constexpr int kCombineThreads = 256;  // the most summaries per combine workgroup
 // Threads of a combine workgroup: the tree's upper levels pick their mode from the workgroup's rows
 // and wavefronts, so 512 threads (32 rows, 8 wavefronts) run the level of 32 merges one per row and
 // the last four one per wavefront (limb-sliced) instead of one per thread and per row. At 228
@@ -1122,18 +1126,18 @@ __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in
   }
 }
 
-// The `count` summaries of the previous level, 256 per workgroup, merged by a tree. The non-empty
+// The `count` summaries of the previous level, `group` per workgroup, merged by a tree. The non-empty
 // summaries are packed to the front in their order first (an empty summary holds no entries, so
 // the ones on either side of it are neighbours), and the tree is as wide as they are many: past the
 // boundary passes most summaries are empty, and a workgroup holding a few pieces of a hot bucket's
 // chain then merges them in a few levels instead of eight.
 __global__ void __launch_bounds__(kCombineLanes) k_combine(const RunSummary* __restrict__ in, int64_t count,
                                                            BucketRec* __restrict__ buckets, RunSummary* __restrict__ out,
-                                                           int last_level) {
+                                                           int last_level, int group) {
   __shared__ RunSummary sh[kCombineThreads];
   __shared__ uint32_t wcount[kCombineThreads / 64];
-  const int64_t t = (int64_t)blockIdx.x * kCombineThreads + threadIdx.x;
-  const bool keep = threadIdx.x < kCombineThreads && t < count && in[t].flags != 0;
+  const int64_t t = (int64_t)blockIdx.x * group + threadIdx.x;
+  const bool keep = (int)threadIdx.x < group && t < count && in[t].flags != 0;
   const uint64_t mask = __ballot(keep);
   const int wave = (int)threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0 && wave < kCombineThreads / 64) wcount[wave] = (uint32_t)__popcll(mask);
@@ -1159,7 +1163,7 @@ __global__ void __launch_bounds__(kCombineLanes) k_combine(const RunSummary* __r
   // next level merges only what continues. A chain's pieces sit at consecutive, non-empty indices,
   // so a run continues only into the summary right outside the range with the same key there.
   if (s->flags & kValid) {
-    const int64_t base = (int64_t)blockIdx.x * kCombineThreads;
+    const int64_t base = (int64_t)blockIdx.x * group;
     const bool single = s->flags & kSingle;
     const uint32_t last = single ? s->fk : s->lk;
     bool cl = false, cr = false;
@@ -1167,8 +1171,8 @@ __global__ void __launch_bounds__(kCombineLanes) k_combine(const RunSummary* __r
       const uint4 m = *reinterpret_cast<const uint4*>(in + base - 1);  // fk, lk, flags of the left neighbour
       cl = (m.z & kValid) && ((m.z & kSingle) ? m.x : m.y) == s->fk;
     }
-    if (base + kCombineThreads < count) {
-      const uint4 m = *reinterpret_cast<const uint4*>(in + base + kCombineThreads);
+    if (base + group < count) {
+      const uint4 m = *reinterpret_cast<const uint4*>(in + base + group);
       cr = (m.z & kValid) && m.x == last;
     }
     if (single) {
@@ -1489,6 +1493,7 @@ struct MsmPlan {
   bool prep_phi = false;
   void* digits = nullptr;
   bool d16 = true;  // 16-bit digit codes (every job's window at most 16 bits), else 32-bit
+  int group = kCombineThreads;  // summaries per combine workgroup
   const AffRec* points = nullptr;
   // device buffers
   void* mem = nullptr;
@@ -1591,7 +1596,9 @@ static int msm_plan_geometry(MsmPlan* plan, std::vector<MsmJob>& jobs, int chunk
   // summaries per workgroup, until a single workgroup remains. Levels alternate between two buffers.
   plan->acc_blocks = grid0;
   plan->level_max.assign(1, grid0 * kAccThreads);
-  while (plan->level_max.back() > 1) plan->level_max.push_back((plan->level_max.back() + kCombineThreads - 1) / kCombineThreads);
+  const int64_t lvl0 = grid0 * kAccThreads;
+  plan->group = lvl0 <= keys || (lvl0 + 63) / 64 <= device_cus() ? 64 : kCombineThreads;
+  while (plan->level_max.back() > 1) plan->level_max.push_back((plan->level_max.back() + plan->group - 1) / plan->group);
   int64_t slots[2] = {1, 1};
   for (size_t l = 0; l < plan->level_max.size(); ++l) slots[l & 1] = std::max(slots[l & 1], plan->level_max[l]);
   // the boundary pass before the first tree level reads level 0 from one buffer and writes as many
@@ -1839,7 +1846,7 @@ static int msm_plan_launch_kernels(MsmPlan* plan, hipStream_t stream) {
       const int lanes = kCombineLanes;
       hipLaunchKernelGGL(k_combine, dim3((unsigned)plan->level_max[(size_t)l]), dim3(lanes), 0, stream,
                          plan->runs[cur], plan->level_max[(size_t)l - 1], plan->buckets, plan->runs[cur ^ 1],
-                         l + 1 == levels ? 1 : 0);
+                         l + 1 == levels ? 1 : 0, plan->group);
       ZK_LAUNCH_CHECK("msm combine");
       cur ^= 1;
     }
