@@ -107,7 +107,7 @@ static void* convert_span(void* arg) {
   return NULL;
 }
 
-enum { kMaxThreads = 16, kMinPerThread = 1 << 15 };
+enum { kMaxThreads = 16, kMinPerThread = 1 << 13 };
 
 static int ingest_threads(Py_ssize_t n) {
   long cpus = sysconf(_SC_NPROCESSORS_ONLN);
@@ -117,6 +117,74 @@ static int ingest_threads(Py_ssize_t n) {
   Py_ssize_t t = n / kMinPerThread;
   if (t > cpus) t = cpus;
   return t < 1 ? 1 : (int)t;
+}
+
+/* Worker threads are started once and kept, so that a caller converting in parts (one part
+ * uploaded while the next is converted) pays a wake-up per part rather than thread creations.
+ * A job is published under the lock with a new generation number; every started worker runs its
+ * span (an empty one past the job's span count) and reports back. The caller holds the
+ * interpreter lock throughout, so jobs never overlap. A forked child has none of the parent's
+ * workers: the pool starts empty there. */
+static struct {
+  pthread_mutex_t mu;
+  pthread_cond_t go, done;
+  int workers;         /* started, ids 1..workers */
+  unsigned long gen;   /* the current job */
+  int pending;         /* workers still on the current job */
+  int nspans;
+  Span spans[kMaxThreads];
+  unsigned long born[kMaxThreads]; /* the generation before a worker's first job */
+} g_pool = {PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER, PTHREAD_COND_INITIALIZER, 0, 0, 0, 0, {{0}}, {0}};
+
+static void* pool_worker(void* arg) {
+  const int id = (int)(intptr_t)arg;
+  pthread_mutex_lock(&g_pool.mu);
+  unsigned long seen = g_pool.born[id]; /* the job published right after its creation is new to it */
+  for (;;) {
+    while (g_pool.gen == seen) pthread_cond_wait(&g_pool.go, &g_pool.mu);
+    seen = g_pool.gen;
+    Span span = g_pool.spans[id];
+    const int active = id < g_pool.nspans;
+    pthread_mutex_unlock(&g_pool.mu);
+    if (active) convert_span(&span);
+    pthread_mutex_lock(&g_pool.mu);
+    if (--g_pool.pending == 0) pthread_cond_signal(&g_pool.done);
+  }
+  return NULL;
+}
+
+static void pool_after_fork_in_child(void) {
+  pthread_mutex_init(&g_pool.mu, NULL);
+  pthread_cond_init(&g_pool.go, NULL);
+  pthread_cond_init(&g_pool.done, NULL);
+  g_pool.workers = 0;
+  g_pool.pending = 0;
+}
+
+/* Runs spans[0, nt) with the calling thread taking span 0; returns once every span is done. */
+static void pool_run(Span* spans, int nt) {
+  pthread_mutex_lock(&g_pool.mu);
+  while (g_pool.workers < nt - 1) {
+    /* a new worker may first take the lock after this job is published: it starts from the
+       generation before it, so the job counts as new to it */
+    pthread_t tid;
+    g_pool.born[g_pool.workers + 1] = g_pool.gen;
+    if (pthread_create(&tid, NULL, pool_worker, (void*)(intptr_t)(g_pool.workers + 1)) != 0) break;
+    pthread_detach(tid);
+    ++g_pool.workers;
+  }
+  const int nw = g_pool.workers;
+  const int run = nt - 1 < nw ? nt - 1 : nw; /* spans the workers take; the rest stay undone */
+  for (int t = 1; t <= run; ++t) g_pool.spans[t] = spans[t];
+  g_pool.nspans = run + 1;
+  g_pool.pending = nw;
+  ++g_pool.gen;
+  pthread_cond_broadcast(&g_pool.go);
+  pthread_mutex_unlock(&g_pool.mu);
+  convert_span(&spans[0]);
+  pthread_mutex_lock(&g_pool.mu);
+  while (g_pool.pending > 0) pthread_cond_wait(&g_pool.done, &g_pool.mu);
+  pthread_mutex_unlock(&g_pool.mu);
 }
 
 /* Converts the n items into rows buf + 32 * stride * i; 0, or -1 with the exception set. */
@@ -129,15 +197,10 @@ static int convert_items(PyObject** it, Py_ssize_t n, PyObject* modulus, const c
   }
   const int nt = ingest_threads(n);
   Span spans[kMaxThreads];
-  pthread_t tids[kMaxThreads];
-  int started = 0;
-  for (int t = 0; t < nt; ++t) {
-    spans[t] = (Span){it, buf, stride, done, n * t / nt, n * (t + 1) / nt};
-    if (t > 0 && pthread_create(&tids[t], NULL, convert_span, &spans[t]) == 0) ++started;
-    else if (t > 0) break; /* fewer workers: the calling thread's second pass converts the rest */
-  }
-  convert_span(&spans[0]);
-  for (int t = 1; t <= started; ++t) pthread_join(tids[t], NULL);
+  for (int t = 0; t < nt; ++t) spans[t] = (Span){it, buf, stride, done, n * t / nt, n * (t + 1) / nt};
+  if (nt > 1) pool_run(spans, nt);
+  else convert_span(&spans[0]);
+  /* what no worker converted (other objects, values to reduce, spans without a worker) */
   for (Py_ssize_t i = 0; i < n; ++i) {
     if (done[i]) continue;
     if (slow_row(it[i], modulus, name, buf + 32 * stride * i) < 0) {
@@ -260,4 +323,8 @@ static PyMethodDef kMethods[] = {
 
 static struct PyModuleDef kModule = {PyModuleDef_HEAD_INIT, "_ingest", NULL, -1, kMethods};
 
-PyMODINIT_FUNC PyInit__ingest(void) { return PyModule_Create(&kModule); }
+PyMODINIT_FUNC PyInit__ingest(void) {
+  static int registered = 0;
+  if (!registered && pthread_atfork(NULL, NULL, pool_after_fork_in_child) == 0) registered = 1;
+  return PyModule_Create(&kModule);
+}

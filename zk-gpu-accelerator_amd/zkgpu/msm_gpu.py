@@ -179,26 +179,39 @@ def _as_sequence(values: Any) -> Sequence[Any]:
 def _msm_staged(points_x: Any, points_y: Any, scalars: Any, window: int) -> Point:
     """msm_pippenger_gpu of host integers. C threads convert each input straight into one
     page-locked block (x and y interleaved as (n, 2, 4) rows, then the scalar rows); the points go
-    up and are reduced, brought into Montgomery form and checked on the curve while the scalars are
-    converted, and the scalars go up in parts as they are converted. The errors and their order are
-    the reference's (msm_gpu.py:432-442): integer types of x, y and the scalars, the lengths, then
-    the first point off the curve."""
+    up in parts as they are converted, and are reduced, brought into Montgomery form and checked
+    on the curve while the scalars are converted, which go up in parts too. The errors and their
+    order are the reference's (msm_gpu.py:432-442): integer types of x, y and the scalars, the
+    lengths, then the first point off the curve."""
     xs, ys = _as_sequence(points_x), _as_sequence(points_y)
     limbs = isinstance(scalars, (DeviceArray, np.ndarray)) and getattr(scalars, "dtype", None) == np.uint64
     ss = scalars if limbs else _as_sequence(scalars)
     m = max(len(xs), len(ys))
     with _gb.PinnedBuffer(64 * m + (0 if limbs else 32 * len(ss))) as buf:
-        _ingest.rows_into(xs, _P, "points_x", buf.view, 2, 0)
-        _ingest.rows_into(ys, _P, "points_y", buf.view, 2, 1)
-        if limbs and (ss.ndim != 2 or ss.shape[1] != _FIELD_LIMBS):
-            raise TypeError(f"scalars must be an (n, 4) uint64 array of plain limbs, got shape {ss.shape}")
-        n = int(ss.shape[0]) if limbs else len(ss)
+        shapes_ok = not limbs or (ss.ndim == 2 and ss.shape[1] == _FIELD_LIMBS)
+        n = (int(ss.shape[0]) if shapes_ok else 0) if limbs else len(ss)
         points = bad = sc = None
-        if len(xs) == len(ys) == n > 0:
-            # the points' device work runs while the scalars are converted
+        raw = None
+        if shapes_ok and len(xs) == len(ys) == n > 0:
+            # the points go up in parts, each while the next one is converted
             _kernels()
             raw = _gb.empty((2 * n, _FIELD_LIMBS))
-            buf.upload(raw, 0, 64 * n)
+            part = max(1 << 16, -(-n // 4))
+            try:
+                for lo in range(0, n, part):
+                    hi = min(n, lo + part)
+                    _ingest.rows_into(xs, _P, "points_x", buf.view, 2, 0, lo, hi)
+                    _ingest.rows_into(ys, _P, "points_y", buf.view, 2, 1, lo, hi)
+                    buf.upload(raw[2 * lo:2 * hi], 64 * lo, 64 * (hi - lo))
+            except TypeError:
+                raw = None  # a non-integer: reported below in the reference's order, all of x first
+        if raw is None:
+            _ingest.rows_into(xs, _P, "points_x", buf.view, 2, 0)
+            _ingest.rows_into(ys, _P, "points_y", buf.view, 2, 1)
+        if not shapes_ok:
+            raise TypeError(f"scalars must be an (n, 4) uint64 array of plain limbs, got shape {ss.shape}")
+        if raw is not None:
+            # the points' device work runs while the scalars are converted
             points = BN254_FQ.from_limbs(raw).reshape(n, 2, _FIELD_LIMBS)
             bad = _gb.empty((1,))
             _gb.call("zk_g1_on_curve_async", points.ptr, n, bad.ptr, _gb.current_stream())
