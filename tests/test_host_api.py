@@ -222,3 +222,32 @@ def test_rows_into_writes_strided_rows_and_parts():
         fg._ingest.rows_into(vals, r, "scalars", bytearray(32 * len(vals)), 2, 0)
     with pytest.raises(TypeError, match=r"^points_x must contain integers: 'float' object"):
         fg._ingest.rows_into([1, 2.5], r, "points_x", bytearray(64), 1, 0)
+
+
+def test_ingest_worker_pool_across_jobs_and_a_fork():
+    """The conversion's worker threads are kept between calls: jobs of every size in a row (one
+    thread, a few, all sixteen), values the workers leave to the calling thread mixed in, and a
+    conversion in a forked child, where the parent's workers do not exist, all give every row."""
+    import os
+    import random
+
+    from zkgpu import field256_gpu as fg
+
+    r = fg.BN254_SCALAR_MODULUS
+    rng = random.Random(11)
+
+    def check(n):
+        vals = [rng.getrandbits(rng.choice((5, 200, 256))) for _ in range(n)]
+        for i in range(0, n, 997):
+            vals[i] = -vals[i] - 1  # reduced by the calling thread
+        buf, _ = fg._ingest.rows_from_ints(vals, r, "v")
+        return bytes(buf) == b"".join((v if v >= 0 else v % r).to_bytes(32, "little") for v in vals)
+
+    for n in (1, 8191, 8192, 40000, 3, 140000, 9000, 0, 140000):
+        assert check(n), n
+    pid = os.fork()
+    if pid == 0:
+        os._exit(0 if check(140000) and check(20000) else 1)
+    _, status = os.waitpid(pid, 0)
+    assert os.WIFEXITED(status) and os.WEXITSTATUS(status) == 0
+    assert check(140000)
