@@ -8,8 +8,9 @@
 //              |k1|, |k2| < 2^127 (GLV: a lattice basis of {(a, b): a + b lambda
 //              = 0 mod r}, zk_constants.hpp), and |k1| (for P_i) and |k2| (for
 //              phi(P_i)) are recoded into W = floor(127 / c) + 1 signed digits of
-//              c bits, d in [-2^(c-1), 2^(c-1)], the signs of k1, k2 folded in. A
-//              digit d != 0 sends +-P to bucket (window, |d|). Against the
+//              c bits, d in [-(2^(c-1) - 1), 2^(c-1)] with the signs of k1, k2 folded
+//              in, stored as 16-bit codes for c <= 16. A digit d != 0 sends +-P to
+//              bucket (window, |d|). Against the
 //              reference's unsigned digits of the full scalar (src/zkgpu/msm_gpu.py
 //              :89-105) this halves the buckets per window and the windows, so the
 //              bucket reduction and the final combination are a quarter and a half
@@ -18,7 +19,8 @@
 //              Pass A sorts the digits by partition (a window's buckets in groups of
 //              512): LDS histograms per (window, tile of scalars), per-partition
 //              offsets, and placement of packed (key, point index | sign) records
-//              from LDS cursors, in runs of hundreds. Pass B sorts each partition by
+//              from LDS cursors, a batch of digits ordered by partition in LDS before
+//              its records leave in runs. Pass B sorts each partition by
 //              bucket in chunks of 8192 records, staged and sorted in LDS and written
 //              one bucket run at a time. Witness-like scalars put a large share of all
 //              entries into a few buckets (booleans send ~15% of them to bucket 1 of
@@ -32,8 +34,9 @@
 //              iterations, so a conversion there would cost the whole wave 4 products
 //              per entry). The runs cut by range boundaries leave as one run summary
 //              per thread; boundary passes join the fragments of short chains, and
-//              trees of 256 summaries per workgroup merge the rest, level after level,
-//              each merge at most one addition. The load is balanced whatever the
+//              trees of 64 or 256 summaries per workgroup merge the rest, level after
+//              level, each merge at most one addition, each level writing the runs
+//              that end inside its workgroups. The load is balanced whatever the
 //              bucket sizes, with no host sync (the reference splits buckets into
 //              chunks level by level and syncs the host per level,
 //              msm_gpu.py:287-333).
