@@ -359,6 +359,27 @@ def test_public_int_path_checks_in_the_reference_order():
         msm_pippenger_gpu([G.x, G.x], [G.y, G.y], np.zeros((1, 4), dtype=np.uint64))
 
 
+def test_public_int_path_order_across_conversion_parts():
+    """Long inputs are converted and uploaded in parts of at least 2^16 points, x and y part by
+    part: a bad y in the first part and a bad x in the second still report x first, as the
+    reference checks all of x before y; then a bad scalar after good points, and a clean call."""
+    n = (1 << 17) + 5
+    xs, ys = [G.x] * n, [G.y] * n
+    bad_x, bad_y = list(xs), list(ys)
+    bad_x[100000] = 2.5
+    bad_y[10] = "y"
+    with pytest.raises(TypeError, match="points_x must contain integers"):
+        msm_pippenger_gpu(bad_x, bad_y, [1] * n)
+    with pytest.raises(TypeError, match="points_y must contain integers"):
+        msm_pippenger_gpu(xs, bad_y, [1] * n)
+    sc = [1] * n
+    sc[n - 1] = None
+    with pytest.raises(TypeError, match="scalars must contain integers"):
+        msm_pippenger_gpu(xs, ys, sc)
+    sc[n - 1] = 3
+    assert msm_pippenger_gpu(xs, ys, sc).affine() == (G * (n + 2)).affine()
+
+
 def test_public_int_path_input_kinds():
     # generators, object arrays, unreduced and negative coordinates, scalars of every range, and
     # scalars as plain uint64 limbs: always (sum s_i k_i mod r) G
