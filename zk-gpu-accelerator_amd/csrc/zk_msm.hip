@@ -338,10 +338,11 @@ __device__ __forceinline__ int find_job(const MsmJob* jobs, int njobs, int64_t x
 }
 
 // A signed digit d as the sort reads it: (|d| - 1) with the sign in the top bit, every bit set
-// for d = 0. Stored digits lie in [-(2^(c-1) - 1), 2^(c-1)] (a GLV half's sign folded in, its ties
-// chosen for that), so no nonzero digit has every bit set, and 16-bit codes hold windows of up to
-// 16 bits (a plan with wider ones stores 32-bit codes): the digit array is written once and read
-// twice, and at 2^20 points 16-bit codes move 100 MB less.
+// for d = 0. No nonzero digit's code has every bit set: below 16-bit windows |d| - 1 < 2^14, and
+// for 16-bit windows the stored digits (a GLV half's sign folded in, its ties chosen for that) lie
+// in [-(2^15 - 1), 2^15]. So 16-bit codes hold windows of up to 16 bits (a plan with wider ones
+// stores 32-bit codes): the digit array is written once and read twice, and at 2^20 points 16-bit
+// codes move 100 MB less.
 template <class D>
 __device__ __forceinline__ D digit_code(int d) {
   if (d == 0) return (D)~(D)0;
@@ -383,8 +384,8 @@ __device__ __forceinline__ void signed_digits_block(const MsmJob* __restrict__ j
   uint32_t c1 = 0, c2 = 0;
   for (int win = 0; win < job.windows; ++win) {
     // a negative half's ties go negative below the top window, so that its folded digits stay in
-    // [-(2^(c-1) - 1), 2^(c-1)] like the positive half's (the top window of a half below
-    // 2^126.97 never reaches 2^(c-1) for c = 16, the widest window with 16-bit codes)
+    // [-(2^(c-1) - 1), 2^(c-1)] like the positive half's wherever c = 16 needs it (the top window
+    // of a half below 2^126.97 stays below 2^15 there; narrower windows have room in the codes)
     const bool top = win + 1 == job.windows;
     const int d1 = signed_digit(m1, win, job.c, c1, neg1 && !top), d2 = signed_digit(m2, win, job.c, c2, neg2 && !top);
     d[(int64_t)win * job.nv + i] = digit_code<D>(neg1 ? -d1 : d1);
