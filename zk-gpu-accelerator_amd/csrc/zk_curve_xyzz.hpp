@@ -51,7 +51,8 @@ __device__ __forceinline__ Xyzz xyzz_dbl_affine(const F& f, const Aff& q) {
 // ZZ3 = ZZ1 PP, ZZZ3 = ZZZ1 PPP. P = 0 doubles (R = 0) or gives the identity.
 // Lazily reduced: P's coordinates below 2p (the identity has ZZ exactly 0, and a finite point's ZZ
 // is never 0 mod p), products without their final subtraction, R and Q - X3 as unreduced operands,
-// X3 from one signed combination; the result's coordinates are below 2p.
+// X3 from one signed combination; the result's coordinates are below 2p. Independent products
+// run in pairs (mont_raw2, sqr_raw2): U2 with S2, PP with R^2, PPP with Q, ZZ3 with ZZZ3.
 // With neg the point added is -(x2, y2): the sign goes into R = -S2 - Y1 (+ 4p), and an identity
 // P becomes (x2, y2, 1, -1), the same point as (x2, -y2, 1, 1) since ZZ^3 = ZZZ^2 still holds,
 // so no coordinate is negated.
@@ -65,8 +66,8 @@ __device__ __forceinline__ Xyzz xyzz_madd(const F& f, const Xyzz& p, const Aff& 
     r.zzz = neg ? fe_const<F>(f.mone) : r.zz;
     return r;
   }
-  const Fe u2 = mont_raw(f, q.x, p.zz);
-  const Fe s2 = mont_raw(f, q.y, p.zzz);
+  Fe u2, s2;
+  mont_raw2(f, q.x, p.zz, q.y, p.zzz, u2, s2);
   const Fe pp0 = fsub_lazy(f, u2, p.x);  // [0, 2p)
   if (fe_is_zero(pp0) || fe_eq(pp0, fe_const<F>(f.p))) {
     Aff qs = q;
@@ -75,11 +76,11 @@ __device__ __forceinline__ Xyzz xyzz_madd(const F& f, const Xyzz& p, const Aff& 
                                                                                         : xyzz_identity(f);
   }
   const Fe rr = fsub_norm_signed(f, s2, p.y, neg);  // +-S2 - Y1 + 2p or 4p, in (0, 4p]
-  const Fe pp = sqr_raw(f, pp0);
-  const Fe ppp = mont_raw(f, pp0, pp);
-  const Fe qq = mont_raw(f, p.x, pp);
+  Fe pp, rr2, ppp, qq;
+  sqr_raw2(f, pp0, rr, pp, rr2);
+  mont_raw2(f, pp0, pp, p.x, pp, ppp, qq);
   Acc64 xa = lc_start(f, 6);  // R^2 - PPP - 2Q + 6p in (0, 8p)
-  lc_add(xa, 1, sqr_raw(f, rr));
+  lc_add(xa, 1, rr2);
   lc_add(xa, -1, ppp);
   lc_add(xa, -2, qq);
   Xyzz r;
@@ -90,8 +91,7 @@ __device__ __forceinline__ Xyzz xyzz_madd(const F& f, const Xyzz& p, const Aff& 
     // R (Q - X3) + (2p - Y1) PPP with one reduction
     r.y = mont_raw_sum2(f, fsub_for_mul(f, qq, r.x), rr, fsub_norm(f, fe_zero(), p.y), ppp);
   }
-  r.zz = mont_raw(f, p.zz, pp);
-  r.zzz = mont_raw(f, p.zzz, ppp);
+  mont_raw2(f, p.zz, pp, p.zzz, ppp, r.zz, r.zzz);
   return r;
 }
 

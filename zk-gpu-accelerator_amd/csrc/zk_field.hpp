@@ -101,6 +101,42 @@ __device__ __forceinline__ Fe canonical(const F& f, const Fe& a) {
   return r;
 }
 
+// Quotient digits. m_k = t_k (-p^-1) only has to be right mod 2^29 to clear the low 29 bits of
+// column k. Field tags may set kFreeQuotient to keep m_0..m_7 at their full 32 bits (one
+// v_and_b32 a column less). The m p part of a column then reaches (2^32 - 1) sum_j p_j
+// instead of 2^29 sum_j p_j, so a tag may set it only for a modulus whose limbs sum to little
+// (free_quotient_fits: the m p part, a carry below 2^35 and an a b part of 9 * 2^60 stay
+// below 2^64) and only where every product's operands fit that a b part: limbs below 2^30 in
+// both operands, or below 3 * 2^29 against 2^29, or the two-product sum of mont_raw_sum2. The
+// last quotient stays masked, so sum_k m_k 2^(29k) < 2^261 (1 + 2^-26) and a product still
+// returns at most a b / 2^261 + p (1 + 2^-26).
+template <class F, class = void>
+struct FreeQuotient {
+  static constexpr bool value = false;
+};
+template <class F>
+struct FreeQuotient<F, decltype((void)F::kFreeQuotient)> {
+  static constexpr bool value = F::kFreeQuotient;
+};
+
+template <class F>
+constexpr bool free_quotient_fits() {
+  unsigned __int128 sum = 0;
+  for (int i = 0; i < NL; ++i) sum += F::p[i];
+  const unsigned __int128 col = (sum << 32) + ((unsigned __int128)9 << 60) + ((unsigned __int128)1 << 35);
+  return col < ((unsigned __int128)1 << 64);
+}
+
+template <class F>
+__device__ __forceinline__ uint32_t mont_quotient(const F& f, uint64_t t, int k) {
+  const uint32_t m = (uint32_t)t * f.ninv;
+  if constexpr (FreeQuotient<F>::value) {
+    static_assert(free_quotient_fits<F>(), "the modulus' limbs sum too high for unmasked quotients");
+    return k < NL - 1 ? m : m & LMASK;
+  }
+  return m & LMASK;
+}
+
 // x * y * 2^-261 mod p, below 2p: one accumulator per column (fewest instructions).
 template <class F>
 __device__ __forceinline__ Fe mont_raw_serial(const F& f, const Fe& a, const Fe& b) {
@@ -113,7 +149,7 @@ __device__ __forceinline__ Fe mont_raw_serial(const F& f, const Fe& a, const Fe&
     for (int i = 0; i <= k; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
 #pragma unroll
     for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * f.p[k - i];
-    m[k] = ((uint32_t)acc * f.ninv) & LMASK;
+    m[k] = mont_quotient(f, acc, k);
     acc += (uint64_t)m[k] * f.p[0];  // clears the low 29 bits
     acc >>= LB;
   }
@@ -128,6 +164,102 @@ __device__ __forceinline__ Fe mont_raw_serial(const F& f, const Fe& a, const Fe&
   }
   r.l[NL - 1] = (uint32_t)acc;
   return r;
+}
+
+// Keeps v where it is in a chain of multiply-adds: the compiler may not reassociate across it.
+__device__ __forceinline__ void in_order(uint64_t& v) { asm("" : "+v"(v)); }
+
+// Two independent products a b and c d (each as mont_raw_serial) in lockstep. Left alone, the
+// compiler splits each column into two partial sums joined by a 64-bit addition, since a
+// multiply-add that reads the one just before it needs a wait state; here each column is one
+// in-order chain per product and the other product's multiply-add sits between two dependent
+// ones: one instruction a column less per product, at the cost of a few s_nop a product (which
+// the other wavefronts on the SIMD fill). Same limb bounds and results as two mont_raw_serial.
+template <class F>
+__device__ __forceinline__ void mont_raw_pair(const F& f, const Fe& a, const Fe& b, const Fe& c, const Fe& d, Fe& r,
+                                              Fe& s) {
+  uint32_t m[NL], n[NL];
+  uint64_t x = 0, y = 0;
+#pragma unroll
+  for (int k = 0; k < 2 * NL - 1; ++k) {
+    const int lo = k < NL ? 0 : k - (NL - 1), hi = k < NL ? k : NL - 1;
+#pragma unroll
+    for (int i = lo; i <= hi; ++i) {
+      x += (uint64_t)a.l[i] * b.l[k - i];
+      in_order(x);
+      y += (uint64_t)c.l[i] * d.l[k - i];
+      in_order(y);
+    }
+#pragma unroll
+    for (int i = lo; i < (k < NL ? k : NL); ++i) {
+      x += (uint64_t)m[i] * f.p[k - i];
+      in_order(x);
+      y += (uint64_t)n[i] * f.p[k - i];
+      in_order(y);
+    }
+    if (k < NL) {
+      m[k] = mont_quotient(f, x, k);
+      n[k] = mont_quotient(f, y, k);
+      x += (uint64_t)m[k] * f.p[0];  // clears the low 29 bits
+      y += (uint64_t)n[k] * f.p[0];
+    } else {
+      r.l[k - NL] = (uint32_t)x & LMASK;
+      s.l[k - NL] = (uint32_t)y & LMASK;
+    }
+    x >>= LB;
+    y >>= LB;
+  }
+  r.l[NL - 1] = (uint32_t)x;
+  s.l[NL - 1] = (uint32_t)y;
+}
+
+// a^2 and c^2 (each as mont_sqr_raw) in lockstep, as mont_raw_pair.
+template <class F>
+__device__ __forceinline__ void mont_sqr_raw_pair(const F& f, const Fe& a, const Fe& c, Fe& r, Fe& s) {
+  uint32_t m[NL], n[NL], da[NL], dc[NL];
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    da[i] = a.l[i] << 1;
+    dc[i] = c.l[i] << 1;
+  }
+  uint64_t x = 0, y = 0;
+#pragma unroll
+  for (int k = 0; k < 2 * NL - 1; ++k) {
+    const int lo = k < NL ? 0 : k - (NL - 1);
+#pragma unroll
+    for (int i = lo; 2 * i < k; ++i) {
+      x += (uint64_t)a.l[i] * da[k - i];
+      in_order(x);
+      y += (uint64_t)c.l[i] * dc[k - i];
+      in_order(y);
+    }
+    if ((k & 1) == 0) {
+      x += (uint64_t)a.l[k / 2] * a.l[k / 2];
+      in_order(x);
+      y += (uint64_t)c.l[k / 2] * c.l[k / 2];
+      in_order(y);
+    }
+#pragma unroll
+    for (int i = lo; i < (k < NL ? k : NL); ++i) {
+      x += (uint64_t)m[i] * f.p[k - i];
+      in_order(x);
+      y += (uint64_t)n[i] * f.p[k - i];
+      in_order(y);
+    }
+    if (k < NL) {
+      m[k] = mont_quotient(f, x, k);
+      n[k] = mont_quotient(f, y, k);
+      x += (uint64_t)m[k] * f.p[0];
+      y += (uint64_t)n[k] * f.p[0];
+    } else {
+      r.l[k - NL] = (uint32_t)x & LMASK;
+      s.l[k - NL] = (uint32_t)y & LMASK;
+    }
+    x >>= LB;
+    y >>= LB;
+  }
+  r.l[NL - 1] = (uint32_t)x;
+  s.l[NL - 1] = (uint32_t)y;
 }
 
 // (a b + c d) 2^-261 mod p below 2p with one reduction, serial columns: for a < 6p with limbs below
@@ -146,7 +278,7 @@ __device__ __forceinline__ Fe mont_raw_sum2(const F& f, const Fe& a, const Fe& b
     for (int i = 0; i <= k; ++i) acc += (uint64_t)a.l[i] * b.l[k - i] + (uint64_t)c.l[i] * d.l[k - i];
 #pragma unroll
     for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * f.p[k - i];
-    m[k] = ((uint32_t)acc * f.ninv) & LMASK;
+    m[k] = mont_quotient(f, acc, k);
     acc += (uint64_t)m[k] * f.p[0];  // clears the low 29 bits
     acc >>= LB;
   }
@@ -186,7 +318,7 @@ __device__ __forceinline__ Fe mont_raw_ilp(const F& f, const Fe& a, const Fe& b)
 #pragma unroll
     for (int i = 0; i < k; ++i) pm += (uint64_t)m[i] * f.p[k - i];
     uint64_t t = pa[k] + carry + pm;
-    m[k] = ((uint32_t)t * f.ninv) & LMASK;
+    m[k] = mont_quotient(f, t, k);
     t += (uint64_t)m[k] * f.p[0];
     carry = t >> LB;
   }
@@ -235,7 +367,7 @@ __device__ __forceinline__ Fe mont_sqr_raw(const F& f, const Fe& a) {
     if ((k & 1) == 0) acc += (uint64_t)a.l[k / 2] * a.l[k / 2];
 #pragma unroll
     for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * f.p[k - i];
-    m[k] = ((uint32_t)acc * f.ninv) & LMASK;
+    m[k] = mont_quotient(f, acc, k);
     acc += (uint64_t)m[k] * f.p[0];
     acc >>= LB;
   }
@@ -398,6 +530,28 @@ template <class F>
 __device__ __forceinline__ Fe sqr_raw(const F& f, const Fe& a) {
   if constexpr (UsesIlp<F>::value) return mont_raw_ilp(f, a, a);
   else return mont_sqr_raw(f, a);
+}
+
+// r = a b, s = c d (as mont_raw), in lockstep with the serial schedule.
+template <class F>
+__device__ __forceinline__ void mont_raw2(const F& f, const Fe& a, const Fe& b, const Fe& c, const Fe& d, Fe& r, Fe& s) {
+  if constexpr (UsesIlp<F>::value) {
+    r = mont_raw_ilp(f, a, b);
+    s = mont_raw_ilp(f, c, d);
+  } else {
+    mont_raw_pair(f, a, b, c, d, r, s);
+  }
+}
+
+// r = a^2, s = c^2 (as sqr_raw), in lockstep with the serial schedule.
+template <class F>
+__device__ __forceinline__ void sqr_raw2(const F& f, const Fe& a, const Fe& c, Fe& r, Fe& s) {
+  if constexpr (UsesIlp<F>::value) {
+    r = mont_raw_ilp(f, a, a);
+    s = mont_raw_ilp(f, c, c);
+  } else {
+    mont_sqr_raw_pair(f, a, c, r, s);
+  }
 }
 
 // a - b in [0, 2p) for a, b < 2p.

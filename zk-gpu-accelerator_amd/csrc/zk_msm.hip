@@ -74,7 +74,19 @@
 
 namespace zk {
 
-using Fq = FqConst;
+// The base field for this file's kernels, with unmasked quotient digits in the thread-serial
+// products (zk_field.hpp, FreeQuotient). Every product operand here is a coordinate or product
+// below 2p with normalized limbs, a sum of two of them (limbs below 2^30), a p4r difference
+// (limbs below 3 * 2^29) against a normalized operand, the two-product sum, or an lc_carry
+// result of a combination below 18p (top limb below 2^27): none takes an a b column part above
+// 9 * 2^60. q's limbs sum below 2^30.6, so the m p part of a column stays below 2^62.6.
+struct FqMsm : FqConst {
+  static constexpr bool kFreeQuotient = true;
+};
+struct FqMsmIlp : FqIlp {
+  static constexpr bool kFreeQuotient = true;
+};
+using Fq = FqMsm;
 
 // ---- packed records -------------------------------------------------------
 
@@ -858,7 +870,7 @@ enum MergeBy { kByThread, kByRow, kByWave };
 // becomes bounded on both sides. At most one point addition.
 template <MergeBy kBy>
 __device__ __forceinline__ void merge_runs(RunSummary* l, const RunSummary* r, BucketRec* __restrict__ buckets) {
-  const FqIlp f;
+  const FqMsmIlp f;
   const bool lead = kBy == kByThread || (threadIdx.x & (kBy == kByRow ? 15 : 63)) == 0;
   const uint32_t lf = l->flags, rf = r->flags;
   if (!(rf & kValid)) return;
@@ -961,7 +973,7 @@ constexpr int kCombineLanes = 512;
 // range's first and last runs as one Jacobian run summary per thread, merged by k_combine (no tree
 // here: its point additions would raise the kernel's registers past three waves per SIMD).
 // The running bucket sum of the accumulation, in Jacobian (madd-2007-bl) or XYZZ (madd-2008-s)
-// coordinates; F picks the product schedule (serial columns, or FqIlp's independent partial sums).
+// coordinates; F picks the product schedule (serial columns, or FqMsmIlp's independent partial sums).
 template <class F, bool kXyzz>
 struct BucketSum;
 
@@ -1080,7 +1092,7 @@ __global__ void __launch_bounds__(256) k_fixup(const RunSummary* __restrict__ in
     o->flags = 0;
     return;
   }
-  const FqIlp f;
+  const FqMsmIlp f;
   const RunSummary& me = in[t];
   // What goes on to the trees: nothing, the whole summary, or one of its runs as a single run.
   // (The output is written field by field from registers: a RunSummary assembled on the stack
@@ -1215,7 +1227,7 @@ __global__ void __launch_bounds__(256) k_reduce_leaf(const MsmJob* __restrict__ 
   const int seg_len = 1 << job.log_seg;
   const int64_t win = local >> job.log_segs, s = local & ((1LL << job.log_segs) - 1);
   const BucketRec* b = buckets + job.key_base + (win << (job.c - 1)) + s * seg_len;
-  const FqIlp f;
+  const FqMsmIlp f;
   // XYZZ running sums (14 products an addition, no conversion per bucket)
   Xyzz r = xyzz_identity(f), t = xyzz_identity(f);
   for (int i = seg_len - 1; i >= 0; --i) {
@@ -1252,7 +1264,7 @@ __global__ void __launch_bounds__(256) k_reduce_bits(const MsmJob* __restrict__ 
   const int64_t win = row / per;
   const int j = (int)(row % per);
   const int64_t segs = 1LL << job.log_segs;
-  const FqIlp f;
+  const FqMsmIlp f;
   Xyzz acc = xyzz_identity(f);
   const int64_t lo = chunk * kRedChunk;
   if (j < job.log_segs) {
@@ -1449,10 +1461,10 @@ using AccKernel = void (*)(const uint2*, const uint32_t*, const AffRec*, BucketR
 
 static AccKernel acc_kernel(int variant) {
   switch (variant & 3) {
-    case 1: return k_accumulate<FqIlp, false>;
-    case 2: return k_accumulate<FqConst, true>;
-    case 3: return k_accumulate<FqIlp, true>;
-    default: return k_accumulate<FqConst, false>;
+    case 1: return k_accumulate<FqMsmIlp, false>;
+    case 2: return k_accumulate<Fq, true>;
+    case 3: return k_accumulate<FqMsmIlp, true>;
+    default: return k_accumulate<Fq, false>;
   }
 }
 
