@@ -14,6 +14,7 @@ total. These tests check, for every operand form the MSM passes to a product:
 - the quotients the replays take do pass 2^29, so the bound is not vacuous.
 """
 import random
+import zlib
 
 import pytest
 
@@ -147,7 +148,7 @@ def random_operand(rng, form, extreme):
 @pytest.mark.parametrize("extreme", [False, True])
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_replay_residue_and_bound(case, extreme):
-    rng = random.Random(hash((case, extreme)) & 0xFFFF)
+    rng = random.Random(zlib.crc32(case.encode()) * 2 + extreme)
     pairs = CASES[case]
     for _ in range(150):
         ops = [(random_operand(rng, A, extreme), random_operand(rng, B, extreme)) for A, B in pairs]
